@@ -1,0 +1,228 @@
+"""ctypes mirror of include/bdsim_gpu.h and the loaders of the native libraries.
+
+The GPU library (bdsim_amd/csrc -> bdsim_amd/libbdsim_gpu.so) is the product.
+`load_gpu()` raises if it is missing or if no GPU is visible: nothing in this
+package falls back to a CPU implementation.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int32, c_int64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+GPU_LIB_PATH = os.path.join(HERE, "libbdsim_gpu.so")
+
+_dp = POINTER(c_double)
+_ip = POINTER(c_int32)
+
+
+class CFlow(ctypes.Structure):
+    _fields_ = [("constant", c_int32), ("n_times", c_int32), ("kappa0", c_double * 9),
+                ("times", _dp), ("kappas", _dp)]
+
+
+class CBeadForce(ctypes.Structure):
+    _fields_ = [("bead", c_int32), ("constant", c_int32), ("value", c_double * 3),
+                ("n_times", c_int32), ("times", _dp), ("values", _dp)]
+
+
+class CPhys(ctypes.Structure):
+    _fields_ = [("spring_type", c_int32), ("sqrtb", c_double), ("natural_length", c_double),
+                ("ev_type", c_int32), ("zstar", c_double), ("dstar", c_double),
+                ("min_cutoff", c_double), ("max_cutoff", c_double),
+                ("contour_dist_for_ev", c_int32), ("n_phi", c_int32), ("phi", _dp),
+                ("bend_type", c_int32), ("bend_stiffness", c_double),
+                ("flow", CFlow),
+                ("n_external", c_int32), ("external", POINTER(CBeadForce)),
+                ("number_of_beads", c_int32), ("hstar", c_double), ("hi_method", c_int32),
+                ("ncheb_multiplier", c_double), ("fd_err_max", c_double),
+                ("equilibration", c_int32)]
+
+
+class CSim(ctypes.Structure):
+    _fields_ = [("time_start", c_double), ("time_end", c_double), ("dt", c_double),
+                ("implicit_loop_tol", c_double), ("update_center_of_mass", c_int32)]
+
+
+class CRng(ctypes.Structure):
+    _fields_ = [("ix", c_int32), ("iy", c_int32), ("seed", c_int32)]
+
+
+class CCounters(ctypes.Structure):
+    _fields_ = [("steps", c_int64), ("matvecs", c_int64), ("lanczos_iters", c_int64),
+                ("lanczos_cold", c_int64), ("lanczos_warm", c_int64),
+                ("cheb_terms", c_int64), ("cheb_attempts", c_int64),
+                ("fallbacks", c_int64), ("corrector_iters", c_int64),
+                ("nonconverged", c_int64), ("fail_index", c_int32), ("pad_", c_int32),
+                ("fail_value", c_double)]
+
+
+class CStepLog(ctypes.Structure):
+    _fields_ = [("t", c_double), ("lmin", c_double), ("lmax", c_double), ("fd_err", c_double),
+                ("lanczos_cold", c_int32), ("lanczos_steps", c_int32), ("cheb_L", c_int32),
+                ("cheb_attempts", c_int32), ("fallback", c_int32),
+                ("corrector_iters", c_int32)]
+
+
+COUNTER_FIELDS = [f for f, _ in CCounters._fields_ if f != "pad_"]
+LOG_FIELDS = [f for f, _ in CStepLog._fields_]
+
+STATUS_OK = 0
+STATUS_NOT_POSITIVE_DEFINITE = 1
+STATUS_RTSAFE_UNBRACKETED = 2
+STATUS_RTSAFE_MAXIT = 3
+STATUS_EXACT_SQRT = 4
+STATUS_WLCB_SIGMA = 5
+STATUS_NOT_RUN = 6
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+
+def dptr(a):
+    """double* of a C-contiguous float64 numpy array (or NULL for None)."""
+    if a is None:
+        return ctypes.cast(None, _dp)
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return ctypes.cast(None, _ip)
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+class PackedPhys:
+    """A CPhys plus the numpy buffers its pointers refer to (kept alive here)."""
+
+    def __init__(self, phys, nbeads):
+        self.keep = []
+        c = CPhys()
+        sp = phys.spring
+        c.spring_type = int(sp.type)
+        c.sqrtb = float(sp.sqrtb)
+        c.natural_length = float(sp.natural_length)
+        ev = phys.ev
+        c.ev_type = int(ev.type)
+        c.zstar, c.dstar = float(ev.zstar), float(ev.dstar)
+        c.min_cutoff, c.max_cutoff = float(ev.min_cutoff), float(ev.max_cutoff)
+        c.contour_dist_for_ev = int(ev.contour_dist_for_EV)
+        phi = np.ascontiguousarray(np.asarray(ev.phi, dtype=np.float64).ravel())
+        if phi.size and int(ev.type) == 3 and phi.size != nbeads * nbeads:
+            raise ValueError(f"EVParams.phi has {phi.size} entries; an SDK chain of "
+                             f"{nbeads} beads needs {nbeads * nbeads} (row-major N*N)")
+        c.n_phi = int(phi.size)
+        c.phi = dptr(phi) if phi.size else dptr(None)
+        self.keep.append(phi)
+        c.bend_type = int(phys.bend.type)
+        c.bend_stiffness = float(phys.bend.stiffness)
+        fl = phys.flow
+        c.flow.constant = 1 if fl.is_constant() else 0
+        k0 = np.asarray(fl.constant_kappa(), dtype=np.float64).ravel()
+        for i in range(9):
+            c.flow.kappa0[i] = k0[i]
+        ts = np.ascontiguousarray(np.asarray(fl.sample_times(), dtype=np.float64))
+        ks = np.ascontiguousarray(np.asarray(fl.sample_tensors(), dtype=np.float64).ravel())
+        c.flow.n_times = int(ts.size)
+        c.flow.times = dptr(ts) if ts.size else dptr(None)
+        c.flow.kappas = dptr(ks) if ks.size else dptr(None)
+        self.keep += [ts, ks]
+        entries = phys.external.entries()
+        if entries:
+            arr = (CBeadForce * len(entries))()
+            for k, e in enumerate(entries):
+                arr[k].bead = int(e.bead)
+                arr[k].constant = 1 if e.constant else 0
+                for d in range(3):
+                    arr[k].value[d] = float(e.value[d])
+                et = np.ascontiguousarray(np.asarray(e.times, dtype=np.float64))
+                ev_ = np.ascontiguousarray(np.asarray(e.values, dtype=np.float64).ravel())
+                arr[k].n_times = int(et.size)
+                arr[k].times = dptr(et) if et.size else dptr(None)
+                arr[k].values = dptr(ev_) if ev_.size else dptr(None)
+                self.keep += [et, ev_]
+            c.n_external = len(entries)
+            c.external = ctypes.cast(arr, POINTER(CBeadForce))
+            self.keep.append(arr)
+        else:
+            c.n_external = 0
+            c.external = ctypes.cast(None, POINTER(CBeadForce))
+        c.number_of_beads = int(nbeads)
+        c.hstar = float(phys.hstar)
+        c.hi_method = int(phys.hi_method)
+        c.ncheb_multiplier = float(phys.ncheb_multiplier)
+        c.fd_err_max = float(phys.fd_err_max)
+        c.equilibration = 1 if phys.equilibration else 0
+        self.c = c
+
+
+def pack_sim(sim):
+    return CSim(float(sim.time_start), float(sim.time_end), float(sim.dt),
+                float(sim.implicit_loop_tol), 1 if sim.update_center_of_mass else 0)
+
+
+# --------------------------------------------------------------------------
+# the GPU library
+# --------------------------------------------------------------------------
+
+_gpu = None
+
+
+class GPUUnavailable(RuntimeError):
+    """The HIP library is not built, or no MI355X-class GPU is visible."""
+
+
+def _declare_gpu(lib):
+    P = POINTER
+    lib.bdsim_init.argtypes = [c_int]
+    lib.bdsim_last_error.restype = ctypes.c_char_p
+    lib.bdsim_max_beads.restype = c_int
+    lib.bdsim_integrate_batch.argtypes = [c_int, _ip, _dp, P(CPhys), P(CSim), P(CRng), c_int,
+                                          _ip, P(CCounters), P(CStepLog), c_int]
+    lib.bdsim_batch_create.argtypes = [P(c_void_p), c_int, c_int, _dp, P(CPhys), _dp,
+                                       P(CRng), c_int]
+    lib.bdsim_batch_run.argtypes = [c_void_p, P(CSim)]
+    lib.bdsim_batch_sync.argtypes = [c_void_p]
+    lib.bdsim_batch_fetch.argtypes = [c_void_p, _dp, P(CRng), _ip, P(CCounters)]
+    lib.bdsim_batch_stream.argtypes = [c_void_p]
+    lib.bdsim_batch_stream.restype = c_void_p
+    lib.bdsim_batch_destroy.argtypes = [c_void_p]
+    lib.bdsim_batch_destroy.restype = None
+    for name in ("bdsim_total_force_batch", "bdsim_spring_force_batch",
+                 "bdsim_bending_force_batch", "bdsim_ev_force_batch"):
+        getattr(lib, name).argtypes = [c_int, c_int, _dp, P(CPhys), _dp]
+    lib.bdsim_external_force_batch.argtypes = [P(CPhys), c_int, c_double, _dp]
+    lib.bdsim_raw_noise_batch.argtypes = [c_int, c_int, c_int, P(CRng), _dp]
+    lib.bdsim_rpy_apply_batch.argtypes = [c_int, c_int, c_double, c_int, _dp, _dp, _dp]
+    return lib
+
+
+def load_gpu_library():
+    """ctypes handle of libbdsim_gpu.so without touching the GPU (symbol checks)."""
+    if not os.path.exists(GPU_LIB_PATH):
+        raise GPUUnavailable(
+            f"{GPU_LIB_PATH} is not built; run `python __graft_entry__.py` "
+            f"(or `make -C bdsim_amd/csrc`) first")
+    return _declare_gpu(ctypes.CDLL(GPU_LIB_PATH))
+
+
+def load_gpu():
+    """The initialised GPU library. Raises GPUUnavailable -- never falls back."""
+    global _gpu
+    if _gpu is None:
+        lib = load_gpu_library()
+        rc = lib.bdsim_init(int(os.environ.get("BDSIM_DEVICE", "0")))
+        if rc != 0:
+            raise GPUUnavailable(f"bdsim_init failed ({rc}): "
+                                 f"{lib.bdsim_last_error().decode()}")
+        _gpu = lib
+    return _gpu
+
+
+def check(lib, rc):
+    if rc != 0:
+        raise RuntimeError(f"bdsim GPU call failed ({rc}): {lib.bdsim_last_error().decode()}")

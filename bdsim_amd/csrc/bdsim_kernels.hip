@@ -1,0 +1,1485 @@
+// bdsim_kernels.hip -- MI355X (gfx950) kernels of the bdsim ChainIntegrator step
+// and the C ABI of include/bdsim_gpu.h.
+//
+// One workgroup integrates one chain over a whole integrate() segment
+// (integrator.cpp:66-85) without leaving the kernel: the step loop, the flow
+// clock, the spectral warm-start cache and the RNG state all stay on chip.
+// Thread tid owns bead tid. The RPY mobility is never stored: every D.x is a
+// matrix-free pair loop over LDS-staged positions (hydrodynamics.cpp:12-54 +
+// linalg.cpp:81-89 fused). The dense tensor is built only for the Cholesky
+// method and for the rare Chebyshev -> Cholesky fallback (mobility.cpp:71-80).
+//
+// Citations are to IsaacPincus/bdsim.
+#include "bdsim_device.hpp"
+#include "../../include/bdsim_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace bdgpu {
+
+struct KArgs {
+    const TrajParams* tp;
+    const int32_t* list;        // trajectory index per workgroup
+    double* R;                  // packed positions
+    double* ws;                 // packed workspace
+    const double* blob;         // phi / flow / external tables
+    const ExtEntry* ext;
+    RngState* rng;
+    int32_t* status;
+    Counters* cnt;
+    StepLog* log;
+    const uint32_t* jx;         // xorshift jump columns [MAX_BEADS][32]
+    const uint32_t* jy;         // minstd jump multipliers [MAX_BEADS]
+    int32_t sim_override;       // nonzero: use the sim fields below for all
+    int32_t com;
+    double t0, t1, dt, tol;
+};
+
+// Mat-vec operand buffers: two (one barrier per mat-vec) while they are small,
+// one (two barriers) for the large-chain blocks, to stay inside the LDS.
+template <int BLOCK> struct XBufs { static constexpr int value = BLOCK <= 256 ? 2 : 1; };
+
+template <int BLOCK, bool MIXED>
+struct Smem {
+    static constexpr int NW = BLOCK / 64;
+    static constexpr int FB = MIXED ? BLOCK : 1;
+    static constexpr int XB = XBufs<BLOCK>::value;
+    double R[3][BLOCK];         // configuration the mobility is evaluated at
+    double P[3][BLOCK];         // configuration the forces are evaluated at
+    double X[XB][3][BLOCK];     // mat-vec operand
+    double B[4][BLOCK];         // bond-level arrays (unit bonds + lengths)
+    double V[3][BLOCK];         // neighbour exchange
+    float Rf[3][FB];            // mixed form: positions relative to bead 0
+    float Xf[XB][3][FB];        // mixed form: mat-vec operand
+    double Red[2 * NW];
+    double Coef[MAXCHEB + 1];
+    // sqrt of the Chebyshev nodes; for large blocks it lives in V, which is
+    // idle while the coefficients are built
+    double Aux[BLOCK >= 256 ? 1 : MAXCHEB + 1];
+    __device__ double* aux() { return BLOCK >= 256 ? &V[0][0] : Aux; }
+    double Alpha[LANCZOS_COLD];
+    double Beta[LANCZOS_COLD];
+    double Tri[2][3][LANCZOS_COLD];
+    double Scal[16];
+    int32_t ScalI[8];
+};
+
+__device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// hydrodynamics.cpp:91-101
+__device__ inline int sturm_count(int k, const double* a, const double* b, double x) {
+    int count = 0;
+    double d = a[0] - x;
+    if (d < 0.0) ++count;
+    for (int i = 1; i < k; ++i) {
+        if (d == 0.0) d = 1.0e-300;
+        d = (a[i] - x) - b[i - 1] * b[i - 1] / d;
+        if (d < 0.0) ++count;
+    }
+    return count;
+}
+
+// hydrodynamics.cpp:106-136; c, dd, y are LDS rows of this lane.
+__device__ inline void tridiag_eigenvector(int m, const double* a, const double* b, double lambda,
+                                           double* c, double* dd, double* y) {
+    const double scale = (fabs(lambda) < 1.0) ? 1.0 : fabs(lambda);
+    const double shift = lambda - 1.0e-7 * scale;
+    for (int i = 0; i < m; ++i) y[i] = 1.0 / sqrt((double)m);
+    for (int it = 0; it < 2; ++it) {
+        double beta0 = a[0] - shift;
+        if (fabs(beta0) < 1.0e-300) beta0 = 1.0e-300;
+        c[0] = (m > 1 ? b[0] / beta0 : 0.0);
+        dd[0] = y[0] / beta0;
+        for (int i = 1; i < m; ++i) {
+            double beta = (a[i] - shift) - b[i - 1] * c[i - 1];
+            if (fabs(beta) < 1.0e-300) beta = 1.0e-300;
+            c[i] = (i < m - 1 ? b[i] / beta : 0.0);
+            dd[i] = (y[i] - b[i - 1] * dd[i - 1]) / beta;
+        }
+        y[m - 1] = dd[m - 1];
+        for (int i = m - 2; i >= 0; --i) y[i] = dd[i] - c[i] * y[i + 1];
+        double nrm = 0.0;
+        for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
+        nrm = sqrt(nrm);
+        if (!(nrm > 0.0) || !isfinite(nrm)) {
+            for (int i = 0; i < m; ++i) y[i] = 1.0 / sqrt((double)m);
+            return;
+        }
+        for (int i = 0; i < m; ++i) y[i] /= nrm;
+    }
+}
+
+template <int BLOCK, bool MIXED>
+struct Chain {
+    static constexpr int NW = BLOCK / 64;
+    Smem<BLOCK, MIXED>& sm;
+    const TrajParams& P;
+    const KArgs& A;
+    const int N, tid, n3;
+    const bool own;
+    const bool hi;
+    const RpyConsts rc;
+    double* Vb;                 // Lanczos basis [LANCZOS_COLD][3N]
+    double* Dm;                 // dense mobility / Cholesky factor [3N][3N]
+    int par = 0, xpar = 0;
+    Counters cnt;
+
+    __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a)
+        : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
+          hi(p.hstar > 0.0), rc(rpy_consts(p.hstar)) {
+        Vb = A.ws + P.ws_off;
+        Dm = Vb + ws_lanczos_doubles(N);
+        cnt = Counters{};
+    }
+
+    __device__ __forceinline__ double bsum(double v) { return block_sum<NW>(v, sm.Red, par); }
+
+    // Stage the mobility configuration (sm.R) and, for the mixed form, its
+    // FP32 copy relative to bead 0. Caller has written sm.R and synchronised.
+    __device__ void stage_mixed() {
+        if constexpr (MIXED) {
+            for (int d = 0; d < 3; ++d)
+                sm.Rf[d][tid] = own ? (float)(sm.R[d][tid] - sm.R[d][0]) : 0.0f;
+            __syncthreads();
+        }
+    }
+
+    // y = D(sm.R) v   (Diffusion::apply, mobility.cpp:53-56)
+    __device__ void apply(const double v[3], double y[3]) {
+        if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
+        constexpr int XB = XBufs<BLOCK>::value;
+        const int b = XB == 2 ? xpar : 0;
+        xpar ^= 1;
+        if constexpr (XB == 1) __syncthreads();
+        for (int d = 0; d < 3; ++d) sm.X[b][d][tid] = v[d];
+        if constexpr (MIXED)
+            for (int d = 0; d < 3; ++d) sm.Xf[b][d][tid] = (float)v[d];
+        __syncthreads();
+        ++cnt.matvecs;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (own) {
+            if constexpr (!MIXED) {
+                const double rx = sm.R[0][tid], ry = sm.R[1][tid], rz = sm.R[2][tid];
+                for (int nu = 0; nu < N; ++nu) {
+                    const double bx = sm.R[0][nu] - rx, by = sm.R[1][nu] - ry, bz = sm.R[2][nu] - rz;
+                    double om1, om2;
+                    rpy_weights64(rc, bx * bx + by * by + bz * bz, om1, om2);
+                    if (nu == tid) { om1 = 1.0; om2 = 0.0; }
+                    const double xx = sm.X[b][0][nu], xy = sm.X[b][1][nu], xz = sm.X[b][2][nu];
+                    const double t = om2 * (bx * xx + by * xy + bz * xz);
+                    ax += om1 * xx + t * bx;
+                    ay += om1 * xy + t * by;
+                    az += om1 * xz + t * bz;
+                }
+            } else {
+                const float hs = (float)rc.hstar, ihs = (float)rc.inv_hstar, TRPI = (float)rc.TRPI,
+                            TPI = (float)rc.TPI, C1 = (float)rc.C1, C3 = (float)rc.C3,
+                            C7 = (float)rc.C7, R34 = (float)rc.RPI3by4;
+                const float rx = sm.Rf[0][tid], ry = sm.Rf[1][tid], rz = sm.Rf[2][tid];
+                int nu = 0;
+                while (nu < N) {
+                    // FP32 pair terms summed over a run of 8 partners, then
+                    // accumulated in FP64
+                    const int end = min(nu + 8, N);
+                    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                    for (; nu < end; ++nu) {
+                        const float bx = sm.Rf[0][nu] - rx, by = sm.Rf[1][nu] - ry,
+                                    bz = sm.Rf[2][nu] - rz;
+                        float om1, om2;
+                        rpy_weights32(hs, ihs, TRPI, TPI, C1, C3, C7, R34,
+                                      bx * bx + by * by + bz * bz, om1, om2);
+                        if (nu == tid) { om1 = 1.0f; om2 = 0.0f; }
+                        const float xx = sm.Xf[b][0][nu], xy = sm.Xf[b][1][nu], xz = sm.Xf[b][2][nu];
+                        const float t = om2 * (bx * xx + by * xy + bz * xz);
+                        fx += om1 * xx + t * bx;
+                        fy += om1 * xy + t * by;
+                        fz += om1 * xz + t * bz;
+                    }
+                    ax += (double)fx;
+                    ay += (double)fy;
+                    az += (double)fz;
+                }
+            }
+        }
+        y[0] = ax; y[1] = ay; y[2] = az;
+    }
+
+    // Spring force on the beads from the bonds held by threads tid < N-1
+    // (spring.cpp:183-207). Returns the bead force; Fc gets the own bond force.
+    __device__ void spring_bead_force(const double bond[3], double Fc[3], double F[3]) {
+        Fc[0] = Fc[1] = Fc[2] = 0.0;
+        if (tid < N - 1) connector_force(P, bond, Fc);
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) sm.V[d][tid] = Fc[d];
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) {
+            double f = 0.0;
+            if (tid == 0) f = sm.V[d][0];
+            else if (tid < N - 1) f = sm.V[d][tid] - sm.V[d][tid - 1];
+            else if (tid == N - 1) f = (-1.0) * sm.V[d][N - 2];
+            F[d] = f;
+        }
+    }
+
+    // Bending force (bending.cpp:5-43) at the configuration in sm.P.
+    __device__ void bending_force(double F[3]) {
+        F[0] = F[1] = F[2] = 0.0;
+        if (P.bend_type == 0 || N == 2) return;
+        const double C = P.bend_C;
+        __syncthreads();
+        if (tid < N - 1) {
+            double b[3];
+            for (int d = 0; d < 3; ++d) b[d] = sm.P[d][tid + 1] - sm.P[d][tid];
+            double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            if (sq < 1.0e-12) sq = 1.0e-12;
+            const double len = sqrt(sq);
+            for (int d = 0; d < 3; ++d) sm.B[d][tid] = (1.0 / len) * b[d];
+            sm.B[3][tid] = len;
+        }
+        __syncthreads();
+        if (!own) return;
+        const int m = tid;
+        auto u = [&](int k, int d) { return sm.B[d][k]; };
+        auto len = [&](int k) { return sm.B[3][k]; };
+        // bend cosines cs(c) = u_{c+1} . u_c (config.cpp:41-45) for the three
+        // angles this bead takes part in; index c is shifted by 2 below.
+        double csl[3];
+        for (int c = m - 2, i = 0; i < 3; ++c, ++i)
+            csl[i] = (c >= 0 && c < N - 2)
+                         ? u(c + 1, 0) * u(c, 0) + u(c + 1, 1) * u(c, 1) + u(c + 1, 2) * u(c, 2)
+                         : 0.0;
+        auto cs = [&](int c) { return csl[c - (m - 2)]; };
+        double A_[3], B_[3], C_[3], D_[3];
+        auto term = [&](double s, int a, double ca, int bb, double* out) {   // s*(ca*u_a - u_b)
+            for (int d = 0; d < 3; ++d) out[d] = s * (ca * u(a, d) - u(bb, d));
+        };
+        auto term2 = [&](double s, double ca, int a, int bb, double* out) {  // s*((-ca)*u_a + u_b)
+            for (int d = 0; d < 3; ++d) out[d] = s * ((-ca) * u(a, d) + u(bb, d));
+        };
+        if (m == 0) { term(C / len(0), 0, cs(0), 1, F); return; }
+        if (m == N - 1) { term2(C / len(N - 2), cs(N - 3), N - 2, N - 3, F); return; }
+        if (N == 3) {
+            term(1.0 / len(1), 1, cs(0), 0, A_);
+            term2(1.0 / len(0), cs(0), 0, 1, B_);
+            for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d]);
+            return;
+        }
+        if (m == 1) {
+            term(1.0 / len(1), 1, cs(0), 0, A_);
+            term2(1.0 / len(0), cs(0), 0, 1, B_);
+            term(1.0 / len(1), 1, cs(1), 2, D_);
+            for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d] + D_[d]);
+            return;
+        }
+        if (m == N - 2) {
+            term(1.0 / len(N - 2), N - 2, cs(N - 3), N - 3, A_);
+            term2(1.0 / len(N - 3), cs(N - 3), N - 3, N - 2, B_);
+            term2(1.0 / len(N - 3), cs(N - 4), N - 3, N - 4, C_);
+            for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d] + C_[d]);
+            return;
+        }
+        term(1.0 / len(m), m, cs(m - 1), m - 1, A_);
+        term2(1.0 / len(m - 1), cs(m - 1), m - 1, m, B_);
+        term2(1.0 / len(m - 1), cs(m - 2), m - 1, m - 2, C_);
+        term(1.0 / len(m), m, cs(m), m + 1, D_);
+        for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d] + C_[d] + D_[d]);
+    }
+
+    // Excluded-volume force (excluded_volume.cpp:60-89) on the own bead at sm.P:
+    // F_mu = sum_nu -fac(r) (R_nu - R_mu) over |nu - mu| >= cd.
+    __device__ void ev_force(double F[3]) {
+        F[0] = F[1] = F[2] = 0.0;
+        if (P.ev_type == 0 || P.ev_type == 4 || !own) return;
+        const double px = sm.P[0][tid], py = sm.P[1][tid], pz = sm.P[2][tid];
+        const double* phi = A.blob + P.phi_off;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int nu = 0; nu < N; ++nu) {
+            const int sep = nu > tid ? nu - tid : tid - nu;
+            if (sep == 0 || sep < P.cd) continue;
+            const double qx = sm.P[0][nu] - px, qy = sm.P[1][nu] - py, qz = sm.P[2][nu] - pz;
+            double sq = qx * qx + qy * qy + qz * qz;
+            if (sq < 1.0e-12) sq = 1.0e-12;
+            const double r = sqrt(sq);
+            double ph = 0.0;
+            if (P.has_phi) ph = nu > tid ? phi[tid * N + nu] : phi[nu * N + tid];
+            const double fac = ev_factor(P, r, ph);
+            fx += (-fac) * qx;
+            fy += (-fac) * qy;
+            fz += (-fac) * qz;
+        }
+        F[0] = fx; F[1] = fy; F[2] = fz;
+    }
+
+    // non-spring + external force at sm.P and time t (model.hpp:56-64)
+    __device__ void explicit_force(double t, double Fe[3]) {
+        double Fev[3], Fb[3], Fx[3] = {0.0, 0.0, 0.0};
+        ev_force(Fev);
+        bending_force(Fb);
+        if (P.n_ext > 0 && own) external_at(P, A.ext, A.blob, N, tid, t, Fx);
+        for (int d = 0; d < 3; ++d) Fe[d] = (Fev[d] + Fb[d]) + Fx[d];
+    }
+
+    // Dense RPY tensor into Dm (hydrodynamics.cpp:12-54, same entry formula and
+    // operand order), then its lower Cholesky factor in place (linalg.cpp:91-107,
+    // same per-entry sums, computed column by column). Returns false on a
+    // non-positive pivot, with its index/value in cnt.
+    __device__ bool dense_factor() {
+        const int n = n3;
+        const double RPI = 1.7724538509055160273;
+        const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
+                     C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75, hstar = P.hstar;
+        __syncthreads();
+        if (own) {
+            const int mu = tid;
+            for (int nu = 0; nu < N; ++nu) {
+                if (nu == mu) {
+                    for (int p = 0; p < 3; ++p)
+                        for (int q = 0; q < 3; ++q) Dm[(size_t)(3 * mu + p) * n + 3 * nu + q] = p == q ? 1.0 : 0.0;
+                    continue;
+                }
+                const int lo = mu < nu ? mu : nu, hi_ = mu < nu ? nu : mu;
+                double b[3];
+                for (int d = 0; d < 3; ++d) b[d] = sm.R[d][hi_] - sm.R[d][lo];
+                double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+                if (sq < 1.0e-12) sq = 1.0e-12;
+                const double rs = sqrt(sq);
+                const double hsbyrs = hstar / rs, rsbyhs = rs / hstar;
+                const double hsbyrs2 = hsbyrs * hsbyrs;
+                double om1, om2;
+                if (rsbyhs >= TRPI) {
+                    om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
+                    om2 = RPI3by4 * (hsbyrs / (rs * rs)) * (1.0 - TPI * hsbyrs2);
+                } else {
+                    om1 = 1.0 - C3 * rsbyhs;
+                    om2 = C7 * rsbyhs / (rs * rs);
+                }
+                for (int p = 0; p < 3; ++p)
+                    for (int q = 0; q < 3; ++q) {
+                        // entry (3mu+p, 3nu+q); reference index i is the lower bead's
+                        const int i = (mu == lo) ? p : q, j = (mu == lo) ? q : p;
+                        Dm[(size_t)(3 * mu + p) * n + 3 * nu + q] = om2 * b[i] * b[j] + (i == j ? om1 : 0.0);
+                    }
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < n; ++j) {
+            if (tid == 0) {
+                double sum = Dm[(size_t)j * n + j];
+                for (int k = 0; k < j; ++k) sum -= Dm[(size_t)j * n + k] * Dm[(size_t)j * n + k];
+                const bool ok = sum > 0.0;
+                sm.ScalI[4] = ok ? 1 : 0;
+                sm.Scal[15] = sum;
+                if (ok) Dm[(size_t)j * n + j] = sqrt(sum);
+            }
+            __syncthreads();
+            if (!sm.ScalI[4]) {
+                cnt.fail_index = j;
+                cnt.fail_value = sm.Scal[15];
+                return false;
+            }
+            const double Ljj = Dm[(size_t)j * n + j];
+            for (int i = j + 1 + tid; i < n; i += BLOCK) {
+                double sum = Dm[(size_t)i * n + j];
+                for (int k = 0; k < j; ++k) sum -= Dm[(size_t)i * n + k] * Dm[(size_t)j * n + k];
+                Dm[(size_t)i * n + j] = sum / Ljj;
+            }
+            __syncthreads();
+        }
+        return true;
+    }
+
+    // y = L x with the factor in Dm (linalg.cpp:109-117), own rows.
+    __device__ void lower_matvec(const double x[3], double y[3]) {
+        constexpr int XB = XBufs<BLOCK>::value;
+        const int b = XB == 2 ? xpar : 0;
+        xpar ^= 1;
+        if constexpr (XB == 1) __syncthreads();
+        for (int d = 0; d < 3; ++d) sm.X[b][d][tid] = x[d];
+        __syncthreads();
+        y[0] = y[1] = y[2] = 0.0;
+        if (!own) return;
+        const int n = n3;
+        for (int p = 0; p < 3; ++p) {
+            const int i = 3 * tid + p;
+            const double* row = Dm + (size_t)i * n;
+            double s = 0.0;
+            for (int j = 0; j <= i; ++j) s += row[j] * sm.X[b][j % 3][j / 3];
+            y[p] = s;
+        }
+    }
+
+    // m-step Lanczos with full reorthogonalisation (hydrodynamics.cpp:146-220).
+    // Returns the Krylov dimension; Ritz values/vectors of the extremes out.
+    __device__ int lanczos(int m, bool warm, const double seed[3], const double u[3],
+                           double& rmin, double& rmax, double vmin[3], double vmax[3]) {
+        if (m > n3) m = n3;
+        double v[3], vprev[3] = {0.0, 0.0, 0.0}, w[3];
+        for (int d = 0; d < 3; ++d) v[d] = warm ? seed[d] : u[d];
+        double nv = sqrt(bsum(dot3(v, v)));
+        if (!(nv > 0.0)) {
+            for (int d = 0; d < 3; ++d) v[d] = u[d];
+            nv = sqrt(bsum(dot3(v, v)));
+        }
+        for (int d = 0; d < 3; ++d) v[d] /= nv;
+        double b = 0.0;
+        int k = 0;
+        const int n = n3;
+        for (int j = 0; j < m; ++j) {
+            if (own)
+                for (int d = 0; d < 3; ++d) Vb[(size_t)j * n + 3 * tid + d] = v[d];
+            apply(v, w);
+            ++cnt.lanczos_iters;
+            const double a = bsum(dot3(v, w));
+            if (tid == 0) sm.Alpha[j] = a;
+            k = j + 1;
+            for (int d = 0; d < 3; ++d) w[d] -= a * v[d] + b * vprev[d];
+            for (int q = 0; q <= j; ++q) {
+                double uq[3] = {0.0, 0.0, 0.0};
+                if (own)
+                    for (int d = 0; d < 3; ++d) uq[d] = Vb[(size_t)q * n + 3 * tid + d];
+                const double dq = bsum(dot3(w, uq));
+                for (int d = 0; d < 3; ++d) w[d] -= dq * uq[d];
+            }
+            b = sqrt(bsum(dot3(w, w)));
+            if (b < 1.0e-10) break;
+            for (int d = 0; d < 3; ++d) { vprev[d] = v[d]; v[d] = w[d] / b; }
+            if (tid == 0) sm.Beta[j] = b;
+        }
+        __syncthreads();
+        if (k == 1) {
+            rmin = rmax = sm.Alpha[0];
+            for (int d = 0; d < 3; ++d) {
+                const double v0 = own ? Vb[3 * tid + d] : 0.0;
+                vmin[d] = v0; vmax[d] = v0;
+            }
+            return k;
+        }
+        if (tid < 2) {
+            const double* al = sm.Alpha;
+            const double* be = sm.Beta;
+            double lo = al[0], hi_ = al[0];
+            for (int i = 0; i < k; ++i) {
+                const double rad = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < k - 1 ? fabs(be[i]) : 0.0);
+                if (al[i] - rad < lo) lo = al[i] - rad;
+                if (hi_ < al[i] + rad) hi_ = al[i] + rad;
+            }
+            const int target = tid == 0 ? 1 : k;
+            double a1 = lo, b1 = hi_;
+            for (int it = 0; it < 100; ++it) {
+                const double mid = 0.5 * (a1 + b1);
+                if (sturm_count(k, al, be, mid) >= target) b1 = mid; else a1 = mid;
+            }
+            const double r = 0.5 * (a1 + b1);
+            sm.Scal[tid] = r;
+            tridiag_eigenvector(k, al, be, r, sm.Tri[tid][0], sm.Tri[tid][1], sm.Tri[tid][2]);
+        }
+        __syncthreads();
+        rmin = sm.Scal[0];
+        rmax = sm.Scal[1];
+        for (int d = 0; d < 3; ++d) { vmin[d] = 0.0; vmax[d] = 0.0; }
+        if (own) {
+            for (int j = 0; j < k; ++j) {
+                const double ymn = sm.Tri[0][2][j], ymx = sm.Tri[1][2][j];
+                for (int d = 0; d < 3; ++d) {
+                    const double vj = Vb[(size_t)j * n + 3 * tid + d];
+                    vmin[d] += ymn * vj;
+                    vmax[d] += ymx * vj;
+                }
+            }
+        }
+        return k;
+    }
+
+    // Chebyshev coefficients of sqrt on the interval (hydrodynamics.cpp:267-279)
+    // into sm.Coef[0..L]: one coefficient per thread.
+    __device__ void cheb_coefficients(int L, double da, double db) {
+        __syncthreads();
+        double* aux = sm.aux();
+        for (int k = tid; k <= L; k += BLOCK) aux[k] = sqrt(cos(PI * (k + 0.5) / (L + 1)) / da - db / da);
+        __syncthreads();
+        for (int j = tid; j <= L; j += BLOCK) {
+            double s = 0.0;
+            for (int k = 0; k <= L; ++k) s += aux[k] * cos(j * (k + 0.5) * PI / (L + 1));
+            s *= 2.0 / (L + 1);
+            if (j == 0) s /= 2.0;
+            sm.Coef[j] = s;
+        }
+        __syncthreads();
+    }
+
+    __device__ void run(int ti) {
+        double R[3] = {0.0, 0.0, 0.0};
+        if (own)
+            for (int d = 0; d < 3; ++d) R[d] = A.R[P.R_off + 3 * tid + d];
+        int32_t ix = A.rng[ti].ix, iy = A.rng[ti].iy, rseed = A.rng[ti].seed;
+        bool rng_ready = false;
+
+        const bool cheb = hi && P.hi_method == 0;
+        double u[3] = {0.0, 0.0, 0.0}, un = 0.0;
+        if (cheb) {
+            if (own)
+                for (int d = 0; d < 3; ++d) u[d] = sin(0.7654321 * (3 * tid + d + 1)) + 0.5;
+            un = sqrt(bsum(dot3(u, u)));
+        }
+        double seedv[3] = {0.0, 0.0, 0.0};
+        bool seed_valid = false;
+        int status = ST_OK;
+
+        const double t0 = A.sim_override ? A.t0 : P.t0;
+        const double t1 = A.sim_override ? A.t1 : P.t1;
+        const double dt = A.sim_override ? A.dt : P.dt;
+        const double tol = A.sim_override ? A.tol : P.tol;
+        const int com = A.sim_override ? A.com : P.com;
+        const double sqrt_dt = sqrt(dt);
+        const double s2i = 1.0 / sqrt(2.0);
+        StepLog* logp = (A.log && P.log_off >= 0) ? A.log + P.log_off : nullptr;
+        int step = 0;
+
+        for (double t = t0; t <= t1 + dt / 2.0; t += dt) {
+            StepLog lg{};
+            lg.t = t;
+            // recenter (integrator.cpp:88-93): sequential sums as the reference
+            if (com) {
+                __syncthreads();
+                for (int d = 0; d < 3; ++d) sm.P[d][tid] = R[d];
+                __syncthreads();
+                if (tid < 3) {
+                    double s = 0.0;
+                    for (int mu = 0; mu < N; ++mu) s = s + sm.P[tid][mu];
+                    sm.Scal[8 + tid] = (1.0 / N) * s;
+                }
+                __syncthreads();
+                if (own)
+                    for (int d = 0; d < 3; ++d) R[d] = R[d] - sm.Scal[8 + d];
+            }
+            __syncthreads();
+            for (int d = 0; d < 3; ++d) { sm.R[d][tid] = R[d]; sm.P[d][tid] = R[d]; }
+            __syncthreads();
+            stage_mixed();
+
+            // forces at R (integrator.cpp:70-76)
+            double bond[3] = {0.0, 0.0, 0.0};
+            if (tid < N - 1)
+                for (int d = 0; d < 3; ++d) bond[d] = sm.P[d][tid + 1] - R[d];
+            double Fc[3], Fs[3], Fe[3];
+            spring_bead_force(bond, Fc, Fs);
+            explicit_force(t, Fe);
+
+            // mobility at R; Cholesky method factors here (mobility.cpp:39-51)
+            if (hi && P.hi_method == 1) {
+                if (!dense_factor()) { status = ST_NOT_PD; break; }
+            }
+
+            // B.dW (mobility.cpp:58-88)
+            if (!rng_ready) { rng_maybe_init(ix, iy, rseed); rng_ready = true; }
+            double x0[3] = {0.0, 0.0, 0.0};
+            {
+                int32_t nix = ix, niy = iy;
+                if (own) bead_noise(A.jx, A.jy, tid, ix, iy, x0, nix, niy);
+                if (tid == N - 1) { sm.ScalI[0] = nix; sm.ScalI[1] = niy; }
+                __syncthreads();
+                ix = sm.ScalI[0];
+                iy = sm.ScalI[1];
+            }
+            double bx0[3] = {0.0, 0.0, 0.0};
+            if (!hi) {
+                for (int d = 0; d < 3; ++d) bx0[d] = x0[d];
+            } else if (P.hi_method == 1) {
+                lower_matvec(x0, bx0);
+            } else if (P.hi_method == 0) {
+                // spectral bounds (hydrodynamics.cpp:224-264)
+                double vmin[3], vmax[3], rmin, rmax;
+                bool cold = !seed_valid;
+                int k = lanczos(seed_valid ? LANCZOS_WARM : LANCZOS_COLD, seed_valid, seedv, u,
+                                rmin, rmax, vmin, vmax);
+                if (seed_valid && k < 4) {
+                    k = lanczos(LANCZOS_COLD, false, seedv, u, rmin, rmax, vmin, vmax);
+                    cold = true;
+                }
+                if (cold) ++cnt.lanczos_cold; else ++cnt.lanczos_warm;
+                lg.lanczos_cold = cold ? 1 : 0;
+                lg.lanczos_steps = k;
+                {
+                    double s[3];
+                    for (int d = 0; d < 3; ++d) s[d] = vmin[d] + vmax[d] + 0.01 * u[d] / un;
+                    const double sn = sqrt(bsum(dot3(s, s)));
+                    if (sn > 0.0 && isfinite(sn)) {
+                        for (int d = 0; d < 3; ++d) seedv[d] = s[d] / sn;
+                        seed_valid = true;
+                    } else {
+                        seed_valid = false;
+                    }
+                }
+                double lmin = 0.5 * rmin;
+                double lmax = 1.02 * rmax;
+                if (lmin < 1.0e-12) lmin = 1.0e-12;
+                if (lmin >= lmax) lmin = lmax * 1.0e-6;
+                lg.lmin = lmin;
+                lg.lmax = lmax;
+
+                // Chebyshev series with FD control (hydrodynamics.cpp:284-337)
+                double Dx0[3];
+                apply(x0, Dx0);
+                const double x0Dx0 = bsum(dot3(x0, Dx0));
+                int L = (int)(sqrt(lmax / lmin) * P.mult + 0.5) + 1;
+                if (L > MAXCHEB) L = MAXCHEB;
+                if (L < 2) L = 2;
+                const double da = 2.0 / (lmax - lmin);
+                const double db = -(lmax + lmin) / (lmax - lmin);
+                double fd = 0.0;
+                for (;;) {
+                    cheb_coefficients(L, da, db);
+                    double tprev[3], tcur[3], res[3];
+                    for (int d = 0; d < 3; ++d) {
+                        res[d] = sm.Coef[0] * x0[d];
+                        tprev[d] = x0[d];
+                        tcur[d] = da * Dx0[d] + db * x0[d];
+                        res[d] += sm.Coef[1] * tcur[d];
+                    }
+                    for (int l = 2; l <= L; ++l) {
+                        double Dv[3];
+                        apply(tcur, Dv);
+                        for (int d = 0; d < 3; ++d) {
+                            double tn = da * Dv[d] + db * tcur[d];
+                            tn = 2.0 * tn - tprev[d];
+                            tprev[d] = tcur[d];
+                            tcur[d] = tn;
+                            res[d] += sm.Coef[l] * tcur[d];
+                        }
+                    }
+                    ++cnt.cheb_attempts;
+                    cnt.cheb_terms += L;
+                    ++lg.cheb_attempts;
+                    fd = fabs(bsum(dot3(res, res)) - x0Dx0) / x0Dx0;
+                    for (int d = 0; d < 3; ++d) bx0[d] = res[d];
+                    if (fd <= P.fd_tol || L >= MAXCHEB) break;
+                    const int grow = (L / 4 > 1) ? L / 4 : 1;
+                    L = (L + grow < MAXCHEB) ? L + grow : MAXCHEB;
+                }
+                lg.cheb_L = L;
+                lg.fd_err = fd;
+                if (!(fd <= P.fd_tol)) {
+                    ++cnt.fallbacks;
+                    lg.fallback = 1;
+                    if (!dense_factor()) { status = ST_NOT_PD; break; }
+                    lower_matvec(x0, bx0);
+                }
+            } else {
+                status = ST_EXACT_SQRT;
+                break;
+            }
+            double dW[3];
+            for (int d = 0; d < 3; ++d) dW[d] = sqrt_dt * bx0[d];
+
+            double K0[9], K1[9];
+            kappa_at(P, A.blob, t, K0);
+            kappa_at(P, A.blob, t + dt, K1);
+
+            // predictor (integrator.cpp:96-101)
+            double v[3], DF[3], fl0[3], Rp[3];
+            for (int d = 0; d < 3; ++d) v[d] = Fs[d] + Fe[d];
+            apply(v, DF);
+            for (int d = 0; d < 3; ++d) fl0[d] = K0[3 * d] * R[0] + K0[3 * d + 1] * R[1] + K0[3 * d + 2] * R[2];
+            for (int d = 0; d < 3; ++d) Rp[d] = own ? R[d] + dt * (fl0[d] + 0.25 * DF[d]) + s2i * dW[d] : 0.0;
+
+            // upsilon (integrator.cpp:107-119)
+            __syncthreads();
+            for (int d = 0; d < 3; ++d) sm.P[d][tid] = Rp[d];
+            __syncthreads();
+            double Fep[3], flow[3], DFs[3], ups[3];
+            explicit_force(t + dt, Fep);
+            for (int d = 0; d < 3; ++d) {
+                const double f1 = K1[3 * d] * Rp[0] + K1[3 * d + 1] * Rp[1] + K1[3 * d + 2] * Rp[2];
+                flow[d] = 0.5 * (fl0[d] + f1);
+            }
+            apply(Fs, DFs);
+            for (int d = 0; d < 3; ++d) v[d] = Fe[d] + Fep[d];
+            apply(v, DF);
+            for (int d = 0; d < 3; ++d)
+                ups[d] = own ? R[d] + dt * (flow[d] + 0.125 * DFs[d] + 0.125 * DF[d]) + s2i * dW[d] : 0.0;
+
+            // implicit connector solve (integrator.cpp:125-165)
+            __syncthreads();
+            for (int d = 0; d < 3; ++d) sm.V[d][tid] = ups[d];
+            __syncthreads();
+            double dups[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
+            if (tid < N - 1)
+                for (int d = 0; d < 3; ++d) {
+                    dups[d] = sm.V[d][tid + 1] - ups[d];
+                    cb[d] = sm.P[d][tid + 1] - Rp[d];
+                }
+            const double anchor0 = sm.P[0][0], anchor1 = sm.P[1][0], anchor2 = sm.P[2][0];
+            double Rprev[3] = {Rp[0], Rp[1], Rp[2]}, Rc[3] = {0.0, 0.0, 0.0};
+            const int max_iter = 100 * N;
+            int iter = 0;
+            bool fail = false;
+            for (; iter <= max_iter; ++iter) {
+                double Fb[3], DFc[3];
+                spring_bead_force(cb, Fc, Fb);
+                apply(Fb, DFc);
+                ++cnt.corrector_iters;
+                ++lg.corrector_iters;
+                __syncthreads();
+                for (int d = 0; d < 3; ++d) sm.V[d][tid] = DFc[d];
+                __syncthreads();
+                int sst = ST_OK;
+                if (tid < N - 1) {
+                    double g[3];
+                    for (int d = 0; d < 3; ++d)
+                        g[d] = dups[d] + (0.125 * dt) * (sm.V[d][tid + 1] - DFc[d]) + (0.25 * dt) * Fc[d];
+                    sst = solve_connector(g, dt, P.spring_type, P.sqrtb, P.q0, cb);
+                }
+                for (int d = 0; d < 3; ++d) sm.B[d][tid] = cb[d];
+                if (sst != ST_OK) sm.ScalI[2] = sst;
+                const int any_fail = __syncthreads_or(sst != ST_OK);
+                if (any_fail) { status = sm.ScalI[2]; fail = true; break; }
+                if (tid < 3) {
+                    double acc = tid == 0 ? anchor0 : (tid == 1 ? anchor1 : anchor2);
+                    sm.V[tid][0] = acc;
+                    for (int mu = 0; mu < N - 1; ++mu) {
+                        acc = acc + sm.B[tid][mu];
+                        sm.V[tid][mu + 1] = acc;
+                    }
+                }
+                __syncthreads();
+                double dv[3] = {0.0, 0.0, 0.0};
+                if (own)
+                    for (int d = 0; d < 3; ++d) { Rc[d] = sm.V[d][tid]; dv[d] = Rc[d] - Rprev[d]; }
+                const double increment = sqrt(bsum(dot3(dv, dv))) / N;
+                if (increment < tol) break;
+                for (int d = 0; d < 3; ++d) Rprev[d] = Rc[d];
+            }
+            if (fail) break;
+            if (iter > max_iter) ++cnt.nonconverged;
+            for (int d = 0; d < 3; ++d) R[d] = Rc[d];
+            ++cnt.steps;
+            if (logp && step < P.log_cap && tid == 0) logp[step] = lg;
+            ++step;
+        }
+
+        if (own)
+            for (int d = 0; d < 3; ++d) A.R[P.R_off + 3 * tid + d] = R[d];
+        if (tid == 0) {
+            A.rng[ti].ix = ix;
+            A.rng[ti].iy = iy;
+            A.rng[ti].seed = rseed;
+            A.status[ti] = status;
+            A.cnt[ti] = cnt;
+        }
+    }
+};
+
+template <int BLOCK, bool MIXED>
+__global__ void __launch_bounds__(BLOCK) k_integrate(KArgs a) {
+    __shared__ Smem<BLOCK, MIXED> sm;
+    const int ti = a.list[blockIdx.x];
+    const TrajParams P = a.tp[ti];
+    Chain<BLOCK, MIXED> c(sm, P, a);
+    c.run(ti);
+}
+
+// ---- force / noise / mobility kernels for the piecewise entry points -------
+enum : int { FORCE_TOTAL = 0, FORCE_SPRING = 1, FORCE_BENDING = 2, FORCE_EV = 3 };
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_forces(KArgs a, int which) {
+    __shared__ Smem<BLOCK, false> sm;
+    const int ti = blockIdx.x;
+    const TrajParams P = a.tp[0];
+    Chain<BLOCK, false> c(sm, P, a);
+    const int N = P.N, tid = threadIdx.x;
+    const bool own = tid < N;
+    const int64_t off = (int64_t)ti * 3 * N;
+    double R[3] = {0.0, 0.0, 0.0};
+    if (own)
+        for (int d = 0; d < 3; ++d) R[d] = a.R[off + 3 * tid + d];
+    for (int d = 0; d < 3; ++d) { sm.R[d][tid] = R[d]; sm.P[d][tid] = R[d]; }
+    __syncthreads();
+    double bond[3] = {0.0, 0.0, 0.0};
+    if (tid < N - 1)
+        for (int d = 0; d < 3; ++d) bond[d] = sm.P[d][tid + 1] - R[d];
+    double Fc[3], Fs[3], Fev[3], Fb[3], F[3];
+    c.spring_bead_force(bond, Fc, Fs);
+    c.ev_force(Fev);
+    c.bending_force(Fb);
+    for (int d = 0; d < 3; ++d) {
+        if (which == FORCE_SPRING) F[d] = Fs[d];
+        else if (which == FORCE_BENDING) F[d] = Fb[d];
+        else if (which == FORCE_EV) F[d] = Fev[d];
+        else F[d] = Fs[d] + (Fev[d] + Fb[d]);   // bindings/module.cpp:67-76
+    }
+    if (own)
+        for (int d = 0; d < 3; ++d) a.ws[off + 3 * tid + d] = F[d];
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_noise(KArgs a, int N, int n_steps) {
+    __shared__ int32_t s[2];
+    const int ti = blockIdx.x, tid = threadIdx.x;
+    int32_t ix = a.rng[ti].ix, iy = a.rng[ti].iy, seed = a.rng[ti].seed;
+    if (n_steps > 0) rng_maybe_init(ix, iy, seed);
+    for (int st = 0; st < n_steps; ++st) {
+        double X[3];
+        int32_t nix = ix, niy = iy;
+        if (tid < N) {
+            bead_noise(a.jx, a.jy, tid, ix, iy, X, nix, niy);
+            for (int d = 0; d < 3; ++d)
+                a.ws[((int64_t)ti * n_steps + st) * 3 * N + 3 * tid + d] = X[d];
+        }
+        if (tid == N - 1) { s[0] = nix; s[1] = niy; }
+        __syncthreads();
+        ix = s[0];
+        iy = s[1];
+        __syncthreads();
+    }
+    if (tid == 0) { a.rng[ti].ix = ix; a.rng[ti].iy = iy; a.rng[ti].seed = seed; }
+}
+
+template <int BLOCK, bool MIXED>
+__global__ void __launch_bounds__(BLOCK) k_rpy_apply(KArgs a) {
+    __shared__ Smem<BLOCK, MIXED> sm;
+    const int ti = blockIdx.x;
+    const TrajParams P = a.tp[0];
+    Chain<BLOCK, MIXED> c(sm, P, a);
+    const int N = P.N, tid = threadIdx.x;
+    const int64_t off = (int64_t)ti * 3 * N;
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) sm.R[d][tid] = tid < N ? a.R[off + 3 * tid + d] : 0.0;
+    if (tid < N)
+        for (int d = 0; d < 3; ++d) x[d] = a.blob[off + 3 * tid + d];
+    __syncthreads();
+    c.stage_mixed();
+    double y[3];
+    c.apply(x, y);
+    if (tid < N)
+        for (int d = 0; d < 3; ++d) a.ws[off + 3 * tid + d] = y[d];
+}
+
+}  // namespace bdgpu
+
+// =========================================================================
+// host side
+// =========================================================================
+namespace {
+
+using namespace bdgpu;
+
+thread_local std::string g_err;
+std::mutex g_mu;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(BDSIM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        if (hipMalloc(&p, want) != hipSuccess) return fail(BDSIM_ENOMEM, "hipMalloc failed");
+        cap = want;
+        return 0;
+    }
+};
+
+struct Ctx {
+    bool ready = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    uint32_t* jx = nullptr;
+    uint32_t* jy = nullptr;
+    DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log;
+} g;
+
+// Columns of T^(3 mu) for the xorshift (13,17,5) map and 16807^(3 mu) mod M.
+void build_jump_tables(std::vector<uint32_t>& jx, std::vector<uint32_t>& jy) {
+    auto step = [](uint32_t x) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; return x; };
+    jx.assign((size_t)MAX_BEADS * 32, 0);
+    jy.assign(MAX_BEADS, 0);
+    std::vector<uint32_t> cur(32);   // columns of T^k, k = 3 mu
+    for (int i = 0; i < 32; ++i) cur[i] = 1u << i;
+    uint64_t a = 1;
+    const uint64_t M = 2147483647ull;
+    for (int mu = 0; mu < MAX_BEADS; ++mu) {
+        for (int i = 0; i < 32; ++i) jx[(size_t)mu * 32 + i] = cur[i];
+        jy[mu] = (uint32_t)a;
+        for (int s = 0; s < 3; ++s) {
+            for (int i = 0; i < 32; ++i) cur[i] = step(cur[i]);   // T is linear: T(col) per column
+            a = (a * 16807ull) % M;
+        }
+    }
+}
+
+int ensure_init() {
+    if (g.ready) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(BDSIM_ENODEV, "no HIP device visible");
+    if (g.device < 0) g.device = 0;
+    HIP_TRY(hipSetDevice(g.device));
+    HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    std::vector<uint32_t> jx, jy;
+    build_jump_tables(jx, jy);
+    HIP_TRY(hipMalloc(&g.jx, jx.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&g.jy, jy.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(g.jx, jx.data(), jx.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g.jy, jy.data(), jy.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    g.ready = true;
+    return 0;
+}
+
+int block_for(int N) {
+    if (N <= 64) return 64;
+    if (N <= 128) return 128;
+    if (N <= 256) return 256;
+    if (N <= 512) return 512;
+    return 1024;
+}
+
+// Host packing of one trajectory (EV constants exactly as
+// excluded_volume.cpp:14-34, evaluated with the host libm).
+int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double>& blob,
+              std::vector<ExtEntry>& ext, TrajParams& t) {
+    std::memset(&t, 0, sizeof t);
+    if (p.spring_type < 1 || p.spring_type > 7) return fail(BDSIM_EINVAL, "bad spring type");
+    if (p.ev_type < 0 || p.ev_type > 4) return fail(BDSIM_EINVAL, "bad EV type");
+    if (p.hi_method < 0 || p.hi_method > 2) return fail(BDSIM_EINVAL, "bad hi_method");
+    t.N = N;
+    t.spring_type = p.spring_type;
+    t.sqrtb = p.sqrtb;
+    t.natural_length = p.natural_length;
+    t.q0 = p.natural_length / p.sqrtb;
+    t.ev_type = p.ev_type;
+    t.cd = p.contour_dist_for_ev;
+    t.equil = p.equilibration ? 1 : 0;
+    const double z = p.zstar, d = p.dstar;
+    t.zsbyds5 = z / std::pow(d, 5.0);
+    t.pt5bydssq = 0.5 / (d * d);
+    t.LJa = 12.0 * 4.0 * z * std::pow(d, 12.0);
+    t.LJb = 6.0 * 4.0 * z * std::pow(d, 6.0);
+    t.SDKa = 12.0 * 4.0 * std::pow(d, 12.0);
+    t.SDKb = 6.0 * 4.0 * std::pow(d, 6.0);
+    t.Rmin = p.min_cutoff;
+    t.Rcutp = p.max_cutoff;
+    t.RcutpSDK = p.max_cutoff;
+    t.Rcutg = d * std::pow(2.0, 1.0 / 6.0);
+    t.LJ_at_Rmin = t.LJa / std::pow(t.Rmin, 14.0) - t.LJb / std::pow(t.Rmin, 8.0);
+    t.SDK_at_Rmin = t.SDKa / std::pow(t.Rmin, 14.0) - t.SDKb / std::pow(t.Rmin, 8.0);
+    t.has_phi = 0;
+    t.phi_off = 0;
+    if (p.ev_type == BDSIM_EV_SDK && p.n_phi > 0) {
+        if (p.n_phi != N * N || !p.phi) return fail(BDSIM_EINVAL, "phi must hold N*N entries");
+        t.has_phi = 1;
+        t.phi_off = (int64_t)blob.size();
+        blob.insert(blob.end(), p.phi, p.phi + p.n_phi);
+    }
+    t.bend_type = p.bend_type;
+    t.bend_C = p.bend_stiffness;
+    t.flow_const = (p.flow.constant || p.flow.n_times == 0) ? 1 : 0;
+    for (int k = 0; k < 9; ++k) t.kappa0[k] = p.flow.kappa0[k];
+    t.flow_n = t.flow_const ? 0 : p.flow.n_times;
+    t.flow_off = (int64_t)blob.size();
+    if (!t.flow_const) {
+        blob.insert(blob.end(), p.flow.times, p.flow.times + p.flow.n_times);
+        blob.insert(blob.end(), p.flow.kappas, p.flow.kappas + 9 * p.flow.n_times);
+    }
+    t.n_ext = p.n_external;
+    t.ext_off = (int64_t)ext.size();
+    for (int k = 0; k < p.n_external; ++k) {
+        const bdsim_bead_force& e = p.external[k];
+        ExtEntry x{};
+        x.bead = e.bead;
+        x.constant = e.constant ? 1 : 0;
+        x.n_times = e.constant ? 0 : e.n_times;
+        for (int q = 0; q < 3; ++q) x.value[q] = e.value[q];
+        x.off = (int64_t)blob.size();
+        if (!e.constant && e.n_times > 0) {
+            blob.insert(blob.end(), e.times, e.times + e.n_times);
+            blob.insert(blob.end(), e.values, e.values + 3 * e.n_times);
+        }
+        ext.push_back(x);
+    }
+    t.hstar = p.hstar;
+    t.hi_method = p.hi_method;
+    t.mult = p.ncheb_multiplier;
+    t.fd_tol = p.fd_err_max;
+    t.t0 = s.time_start;
+    t.t1 = s.time_end;
+    t.dt = s.dt;
+    t.tol = s.implicit_loop_tol;
+    t.com = s.update_center_of_mass ? 1 : 0;
+    t.log_off = -1;
+    t.log_cap = 0;
+    return 0;
+}
+
+template <bool MIXED>
+int launch_bucket(int block, int count, const KArgs& a) {
+    if (count == 0) return 0;
+    dim3 grid(count), blk(block);
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_integrate<64, MIXED>), grid, blk, 0, g.stream, a); break;
+        case 128: hipLaunchKernelGGL((k_integrate<128, MIXED>), grid, blk, 0, g.stream, a); break;
+        case 256: hipLaunchKernelGGL((k_integrate<256, MIXED>), grid, blk, 0, g.stream, a); break;
+        case 512: hipLaunchKernelGGL((k_integrate<512, MIXED>), grid, blk, 0, g.stream, a); break;
+        default: hipLaunchKernelGGL((k_integrate<1024, MIXED>), grid, blk, 0, g.stream, a); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+T* as(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
+
+}  // namespace
+
+struct bdsim_batch {
+    int n_traj = 0, N = 0, precision = 0, block = 64;
+    hipStream_t stream = nullptr;
+    TrajParams* tp = nullptr;
+    int32_t* list = nullptr;
+    double* R = nullptr;
+    double* ws = nullptr;
+    double* blob = nullptr;
+    ExtEntry* ext = nullptr;
+    RngState* rng = nullptr;
+    int32_t* status = nullptr;
+    Counters* cnt = nullptr;
+};
+
+extern "C" {
+
+int bdsim_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g.ready && device != g.device) return fail(BDSIM_EINVAL, "already initialised on another device");
+    g.device = device;
+    return ensure_init();
+}
+
+const char* bdsim_last_error(void) { return g_err.c_str(); }
+
+int bdsim_max_beads(void) { return MAX_BEADS; }
+
+int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bdsim_phys* phys,
+                          const bdsim_sim* sim, bdsim_rng_state* rng, int precision,
+                          int32_t* status, bdsim_counters* counters, bdsim_step_log* log,
+                          int log_stride) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init()) return rc;
+    if (n_traj < 0 || !nbeads || !R || !phys || !sim || !rng || !status)
+        return fail(BDSIM_EINVAL, "null argument");
+    if (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED)
+        return fail(BDSIM_EINVAL, "bad precision");
+    if (n_traj == 0) return 0;
+    std::vector<TrajParams> tps(n_traj);
+    std::vector<double> blob;
+    std::vector<ExtEntry> ext;
+    int64_t roff = 0, wsoff = 0;
+    for (int i = 0; i < n_traj; ++i) {
+        const int N = nbeads[i];
+        if (N < 2 || N > MAX_BEADS) return fail(BDSIM_EINVAL, "nbeads out of range [2, max_beads]");
+        if (int rc = pack_traj(phys[i], sim[i], N, blob, ext, tps[i])) return rc;
+        tps[i].R_off = roff;
+        tps[i].ws_off = wsoff;
+        roff += 3 * N;
+        if (phys[i].hstar > 0.0) wsoff += ws_lanczos_doubles(N) + ws_dense_doubles(N);
+        if (log && log_stride > 0) {
+            tps[i].log_off = (int64_t)i * log_stride;
+            tps[i].log_cap = log_stride;
+        }
+    }
+    if (blob.empty()) blob.push_back(0.0);
+    if (ext.empty()) ext.push_back(ExtEntry{});
+    // buckets by block size
+    std::vector<int32_t> order;
+    std::vector<int> counts(5, 0);
+    const int blocks[5] = {64, 128, 256, 512, 1024};
+    for (int b = 0; b < 5; ++b)
+        for (int i = 0; i < n_traj; ++i)
+            if (block_for(nbeads[i]) == blocks[b]) { order.push_back(i); ++counts[b]; }
+
+    if (int rc = g.tp.ensure(sizeof(TrajParams) * n_traj)) return rc;
+    if (int rc = g.list.ensure(sizeof(int32_t) * n_traj)) return rc;
+    if (int rc = g.R.ensure(sizeof(double) * roff)) return rc;
+    if (int rc = g.ws.ensure(sizeof(double) * std::max<int64_t>(wsoff, 1))) return rc;
+    if (int rc = g.blob.ensure(sizeof(double) * blob.size())) return rc;
+    if (int rc = g.ext.ensure(sizeof(ExtEntry) * ext.size())) return rc;
+    if (int rc = g.rng.ensure(sizeof(RngState) * n_traj)) return rc;
+    if (int rc = g.status.ensure(sizeof(int32_t) * n_traj)) return rc;
+    if (int rc = g.cnt.ensure(sizeof(Counters) * n_traj)) return rc;
+    const size_t nlog = (log && log_stride > 0) ? (size_t)n_traj * log_stride : 1;
+    if (int rc = g.log.ensure(sizeof(StepLog) * nlog)) return rc;
+
+    static_assert(sizeof(RngState) == sizeof(bdsim_rng_state), "rng layout");
+    static_assert(sizeof(Counters) == sizeof(bdsim_counters), "counter layout");
+    static_assert(sizeof(StepLog) == sizeof(bdsim_step_log), "log layout");
+
+    HIP_TRY(hipMemcpyAsync(g.tp.p, tps.data(), sizeof(TrajParams) * n_traj, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.list.p, order.data(), sizeof(int32_t) * n_traj, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.R.p, R, sizeof(double) * roff, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.blob.p, blob.data(), sizeof(double) * blob.size(), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.ext.p, ext.data(), sizeof(ExtEntry) * ext.size(), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.rng.p, rng, sizeof(RngState) * n_traj, hipMemcpyHostToDevice, g.stream));
+
+    KArgs a{};
+    a.tp = as<TrajParams>(g.tp);
+    a.R = as<double>(g.R);
+    a.ws = as<double>(g.ws);
+    a.blob = as<double>(g.blob);
+    a.ext = as<ExtEntry>(g.ext);
+    a.rng = as<RngState>(g.rng);
+    a.status = as<int32_t>(g.status);
+    a.cnt = as<Counters>(g.cnt);
+    a.log = (log && log_stride > 0) ? as<StepLog>(g.log) : nullptr;
+    a.jx = g.jx;
+    a.jy = g.jy;
+    a.sim_override = 0;
+    int start = 0;
+    for (int b = 0; b < 5; ++b) {
+        a.list = as<int32_t>(g.list) + start;
+        int rc = precision == BDSIM_PRECISION_MIXED ? launch_bucket<true>(blocks[b], counts[b], a)
+                                                    : launch_bucket<false>(blocks[b], counts[b], a);
+        if (rc) return rc;
+        start += counts[b];
+    }
+    HIP_TRY(hipMemcpyAsync(R, g.R.p, sizeof(double) * roff, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(rng, g.rng.p, sizeof(RngState) * n_traj, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(status, g.status.p, sizeof(int32_t) * n_traj, hipMemcpyDeviceToHost, g.stream));
+    if (counters)
+        HIP_TRY(hipMemcpyAsync(counters, g.cnt.p, sizeof(Counters) * n_traj, hipMemcpyDeviceToHost, g.stream));
+    if (log && log_stride > 0)
+        HIP_TRY(hipMemcpyAsync(log, g.log.p, sizeof(StepLog) * nlog, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+// ---- device-resident batches ---------------------------------------------
+
+int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* R0,
+                       const bdsim_phys* phys, const double* kappa, const bdsim_rng_state* rng,
+                       int precision) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init()) return rc;
+    if (!out || n_traj <= 0 || !R0 || !phys || !rng) return fail(BDSIM_EINVAL, "bad argument");
+    if (nbeads < 2 || nbeads > MAX_BEADS) return fail(BDSIM_EINVAL, "nbeads out of range");
+    if (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED)
+        return fail(BDSIM_EINVAL, "bad precision");
+    const int N = nbeads;
+    bdsim_sim s0{0.0, 0.0, 0.01, 1e-3, 1};
+    std::vector<double> blob;
+    std::vector<ExtEntry> ext;
+    TrajParams proto;
+    if (int rc = pack_traj(*phys, s0, N, blob, ext, proto)) return rc;
+    std::vector<TrajParams> tps(n_traj, proto);
+    const int64_t wsper = phys->hstar > 0.0 ? ws_lanczos_doubles(N) + ws_dense_doubles(N) : 0;
+    std::vector<int32_t> list(n_traj);
+    for (int i = 0; i < n_traj; ++i) {
+        tps[i].R_off = (int64_t)i * 3 * N;
+        tps[i].ws_off = (int64_t)i * wsper;
+        if (kappa) {
+            tps[i].flow_const = 1;
+            tps[i].flow_n = 0;
+            for (int k = 0; k < 9; ++k) tps[i].kappa0[k] = kappa[9 * i + k];
+        }
+        list[i] = i;
+    }
+    if (blob.empty()) blob.push_back(0.0);
+    if (ext.empty()) ext.push_back(ExtEntry{});
+    bdsim_batch* b = new bdsim_batch();
+    b->n_traj = n_traj;
+    b->N = N;
+    b->precision = precision;
+    b->block = block_for(N);
+    auto cleanup = [&](int code, const char* what) {
+        bdsim_batch_destroy(b);
+        return fail(code, what);
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(BDSIM_EHIP, "stream create failed");
+    if (hipMalloc(&b->tp, sizeof(TrajParams) * n_traj) != hipSuccess ||
+        hipMalloc(&b->list, sizeof(int32_t) * n_traj) != hipSuccess ||
+        hipMalloc(&b->R, sizeof(double) * 3 * N * (size_t)n_traj) != hipSuccess ||
+        hipMalloc(&b->ws, sizeof(double) * std::max<size_t>(1, (size_t)wsper * n_traj)) != hipSuccess ||
+        hipMalloc(&b->blob, sizeof(double) * blob.size()) != hipSuccess ||
+        hipMalloc(&b->ext, sizeof(ExtEntry) * ext.size()) != hipSuccess ||
+        hipMalloc(&b->rng, sizeof(RngState) * n_traj) != hipSuccess ||
+        hipMalloc(&b->status, sizeof(int32_t) * n_traj) != hipSuccess ||
+        hipMalloc(&b->cnt, sizeof(Counters) * n_traj) != hipSuccess)
+        return cleanup(BDSIM_ENOMEM, "hipMalloc failed for batch");
+    if (hipMemcpy(b->tp, tps.data(), sizeof(TrajParams) * n_traj, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->list, list.data(), sizeof(int32_t) * n_traj, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->R, R0, sizeof(double) * 3 * N * (size_t)n_traj, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->blob, blob.data(), sizeof(double) * blob.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->ext, ext.data(), sizeof(ExtEntry) * ext.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->rng, rng, sizeof(RngState) * n_traj, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(BDSIM_EHIP, "upload failed for batch");
+    *out = b;
+    return 0;
+}
+
+int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
+    if (!b || !sim) return fail(BDSIM_EINVAL, "null argument");
+    KArgs a{};
+    a.tp = b->tp;
+    a.list = b->list;
+    a.R = b->R;
+    a.ws = b->ws;
+    a.blob = b->blob;
+    a.ext = b->ext;
+    a.rng = b->rng;
+    a.status = b->status;
+    a.cnt = b->cnt;
+    a.log = nullptr;
+    a.jx = g.jx;
+    a.jy = g.jy;
+    a.sim_override = 1;
+    a.com = sim->update_center_of_mass ? 1 : 0;
+    a.t0 = sim->time_start;
+    a.t1 = sim->time_end;
+    a.dt = sim->dt;
+    a.tol = sim->implicit_loop_tol;
+    dim3 grid(b->n_traj), blk(b->block);
+    const bool mixed = b->precision == BDSIM_PRECISION_MIXED;
+#define LAUNCH_B(BS)                                                                              \
+    if (mixed) hipLaunchKernelGGL((k_integrate<BS, true>), grid, blk, 0, b->stream, a);           \
+    else hipLaunchKernelGGL((k_integrate<BS, false>), grid, blk, 0, b->stream, a);
+    switch (b->block) {
+        case 64: LAUNCH_B(64); break;
+        case 128: LAUNCH_B(128); break;
+        case 256: LAUNCH_B(256); break;
+        case 512: LAUNCH_B(512); break;
+        default: LAUNCH_B(1024); break;
+    }
+#undef LAUNCH_B
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int bdsim_batch_sync(bdsim_batch* b) {
+    if (!b) return fail(BDSIM_EINVAL, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int bdsim_batch_fetch(bdsim_batch* b, double* R, bdsim_rng_state* rng, int32_t* status,
+                      bdsim_counters* counters) {
+    if (!b) return fail(BDSIM_EINVAL, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (R) HIP_TRY(hipMemcpy(R, b->R, sizeof(double) * 3 * b->N * (size_t)b->n_traj, hipMemcpyDeviceToHost));
+    if (rng) HIP_TRY(hipMemcpy(rng, b->rng, sizeof(RngState) * b->n_traj, hipMemcpyDeviceToHost));
+    if (status) HIP_TRY(hipMemcpy(status, b->status, sizeof(int32_t) * b->n_traj, hipMemcpyDeviceToHost));
+    if (counters) HIP_TRY(hipMemcpy(counters, b->cnt, sizeof(Counters) * b->n_traj, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+void* bdsim_batch_stream(bdsim_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+void bdsim_batch_destroy(bdsim_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)hipFree(b->tp); (void)hipFree(b->list); (void)hipFree(b->R); (void)hipFree(b->ws);
+    (void)hipFree(b->blob); (void)hipFree(b->ext); (void)hipFree(b->rng);
+    (void)hipFree(b->status); (void)hipFree(b->cnt);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// ---- piecewise entry points ------------------------------------------------
+
+static int forces_common(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
+                         double* F, int which) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init()) return rc;
+    if (n_traj <= 0 || nbeads < 2 || nbeads > MAX_BEADS || !R || !phys || !F)
+        return fail(BDSIM_EINVAL, "bad argument");
+    std::vector<double> blob;
+    std::vector<ExtEntry> ext;
+    TrajParams tp;
+    bdsim_sim s0{0.0, 0.0, 0.01, 1e-3, 1};
+    if (int rc = pack_traj(*phys, s0, nbeads, blob, ext, tp)) return rc;
+    if (blob.empty()) blob.push_back(0.0);
+    if (ext.empty()) ext.push_back(ExtEntry{});
+    const size_t nd = (size_t)n_traj * 3 * nbeads;
+    if (int rc = g.tp.ensure(sizeof(TrajParams))) return rc;
+    if (int rc = g.R.ensure(sizeof(double) * nd)) return rc;
+    if (int rc = g.ws.ensure(sizeof(double) * nd)) return rc;
+    if (int rc = g.blob.ensure(sizeof(double) * blob.size())) return rc;
+    if (int rc = g.ext.ensure(sizeof(ExtEntry) * ext.size())) return rc;
+    HIP_TRY(hipMemcpyAsync(g.tp.p, &tp, sizeof tp, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.R.p, R, sizeof(double) * nd, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.blob.p, blob.data(), sizeof(double) * blob.size(), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.ext.p, ext.data(), sizeof(ExtEntry) * ext.size(), hipMemcpyHostToDevice, g.stream));
+    KArgs a{};
+    a.tp = as<TrajParams>(g.tp);
+    a.R = as<double>(g.R);
+    a.ws = as<double>(g.ws);
+    a.blob = as<double>(g.blob);
+    a.ext = as<ExtEntry>(g.ext);
+    dim3 grid(n_traj);
+    switch (block_for(nbeads)) {
+        case 64: hipLaunchKernelGGL(k_forces<64>, grid, dim3(64), 0, g.stream, a, which); break;
+        case 128: hipLaunchKernelGGL(k_forces<128>, grid, dim3(128), 0, g.stream, a, which); break;
+        case 256: hipLaunchKernelGGL(k_forces<256>, grid, dim3(256), 0, g.stream, a, which); break;
+        case 512: hipLaunchKernelGGL(k_forces<512>, grid, dim3(512), 0, g.stream, a, which); break;
+        default: hipLaunchKernelGGL(k_forces<1024>, grid, dim3(1024), 0, g.stream, a, which); break;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(F, g.ws.p, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int bdsim_total_force_batch(int n, int N, const double* R, const bdsim_phys* p, double* F) {
+    return forces_common(n, N, R, p, F, FORCE_TOTAL);
+}
+int bdsim_spring_force_batch(int n, int N, const double* R, const bdsim_phys* p, double* F) {
+    return forces_common(n, N, R, p, F, FORCE_SPRING);
+}
+int bdsim_bending_force_batch(int n, int N, const double* R, const bdsim_phys* p, double* F) {
+    return forces_common(n, N, R, p, F, FORCE_BENDING);
+}
+int bdsim_ev_force_batch(int n, int N, const double* R, const bdsim_phys* p, double* F) {
+    return forces_common(n, N, R, p, F, FORCE_EV);
+}
+
+// External forces are a handful of table lookups; they are evaluated by the
+// same device function the step uses, on one workgroup.
+namespace {
+__global__ void k_external(KArgs a, double t) {
+    const TrajParams P = a.tp[0];
+    const int mu = threadIdx.x + blockIdx.x * blockDim.x;
+    if (mu >= P.N) return;
+    double F[3];
+    external_at(P, a.ext, a.blob, P.N, mu, t, F);
+    for (int d = 0; d < 3; ++d) a.ws[3 * mu + d] = F[d];
+}
+}  // namespace
+
+int bdsim_external_force_batch(const bdsim_phys* phys, int nbeads, double t, double* F) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init()) return rc;
+    if (!phys || !F || nbeads < 1 || nbeads > (1 << 24)) return fail(BDSIM_EINVAL, "bad argument");
+    std::vector<double> blob;
+    std::vector<ExtEntry> ext;
+    TrajParams tp;
+    bdsim_sim s0{0.0, 0.0, 0.01, 1e-3, 1};
+    bdsim_phys p = *phys;
+    p.n_phi = 0;   // phi is irrelevant to the external force
+    if (int rc = pack_traj(p, s0, nbeads, blob, ext, tp)) return rc;
+    if (blob.empty()) blob.push_back(0.0);
+    if (ext.empty()) ext.push_back(ExtEntry{});
+    if (int rc = g.tp.ensure(sizeof(TrajParams))) return rc;
+    if (int rc = g.ws.ensure(sizeof(double) * 3 * nbeads)) return rc;
+    if (int rc = g.blob.ensure(sizeof(double) * blob.size())) return rc;
+    if (int rc = g.ext.ensure(sizeof(ExtEntry) * ext.size())) return rc;
+    HIP_TRY(hipMemcpyAsync(g.tp.p, &tp, sizeof tp, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.blob.p, blob.data(), sizeof(double) * blob.size(), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.ext.p, ext.data(), sizeof(ExtEntry) * ext.size(), hipMemcpyHostToDevice, g.stream));
+    KArgs a{};
+    a.tp = as<TrajParams>(g.tp);
+    a.ws = as<double>(g.ws);
+    a.blob = as<double>(g.blob);
+    a.ext = as<ExtEntry>(g.ext);
+    hipLaunchKernelGGL(k_external, dim3((nbeads + 255) / 256), dim3(256), 0, g.stream, a, t);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(F, g.ws.p, sizeof(double) * 3 * nbeads, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int bdsim_raw_noise_batch(int n_traj, int nbeads, int n_steps, bdsim_rng_state* rng, double* X) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init()) return rc;
+    if (n_traj <= 0 || nbeads < 1 || nbeads > MAX_BEADS || n_steps < 0 || !rng || !X)
+        return fail(BDSIM_EINVAL, "bad argument");
+    const size_t nd = (size_t)n_traj * n_steps * 3 * nbeads;
+    if (int rc = g.ws.ensure(sizeof(double) * std::max<size_t>(nd, 1))) return rc;
+    if (int rc = g.rng.ensure(sizeof(RngState) * n_traj)) return rc;
+    HIP_TRY(hipMemcpyAsync(g.rng.p, rng, sizeof(RngState) * n_traj, hipMemcpyHostToDevice, g.stream));
+    KArgs a{};
+    a.ws = as<double>(g.ws);
+    a.rng = as<RngState>(g.rng);
+    a.jx = g.jx;
+    a.jy = g.jy;
+    dim3 grid(n_traj);
+    switch (block_for(nbeads)) {
+        case 64: hipLaunchKernelGGL(k_noise<64>, grid, dim3(64), 0, g.stream, a, nbeads, n_steps); break;
+        case 128: hipLaunchKernelGGL(k_noise<128>, grid, dim3(128), 0, g.stream, a, nbeads, n_steps); break;
+        case 256: hipLaunchKernelGGL(k_noise<256>, grid, dim3(256), 0, g.stream, a, nbeads, n_steps); break;
+        case 512: hipLaunchKernelGGL(k_noise<512>, grid, dim3(512), 0, g.stream, a, nbeads, n_steps); break;
+        default: hipLaunchKernelGGL(k_noise<1024>, grid, dim3(1024), 0, g.stream, a, nbeads, n_steps); break;
+    }
+    HIP_TRY(hipGetLastError());
+    if (nd) HIP_TRY(hipMemcpyAsync(X, g.ws.p, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(rng, g.rng.p, sizeof(RngState) * n_traj, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision, const double* R,
+                          const double* x, double* y) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = ensure_init()) return rc;
+    if (n_traj <= 0 || nbeads < 1 || nbeads > MAX_BEADS || !R || !x || !y)
+        return fail(BDSIM_EINVAL, "bad argument");
+    if (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED)
+        return fail(BDSIM_EINVAL, "bad precision");
+    TrajParams tp;
+    std::memset(&tp, 0, sizeof tp);
+    tp.N = nbeads;
+    tp.hstar = hstar;
+    const size_t nd = (size_t)n_traj * 3 * nbeads;
+    if (int rc = g.tp.ensure(sizeof(TrajParams))) return rc;
+    if (int rc = g.R.ensure(sizeof(double) * nd)) return rc;
+    if (int rc = g.blob.ensure(sizeof(double) * nd)) return rc;
+    if (int rc = g.ws.ensure(sizeof(double) * nd)) return rc;
+    HIP_TRY(hipMemcpyAsync(g.tp.p, &tp, sizeof tp, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.R.p, R, sizeof(double) * nd, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(g.blob.p, x, sizeof(double) * nd, hipMemcpyHostToDevice, g.stream));
+    KArgs a{};
+    a.tp = as<TrajParams>(g.tp);
+    a.R = as<double>(g.R);
+    a.blob = as<double>(g.blob);
+    a.ws = as<double>(g.ws);
+    dim3 grid(n_traj);
+    const bool mixed = precision == BDSIM_PRECISION_MIXED;
+#define LAUNCH_A(BS)                                                                         \
+    if (mixed) hipLaunchKernelGGL((k_rpy_apply<BS, true>), grid, dim3(BS), 0, g.stream, a);  \
+    else hipLaunchKernelGGL((k_rpy_apply<BS, false>), grid, dim3(BS), 0, g.stream, a);
+    switch (block_for(nbeads)) {
+        case 64: LAUNCH_A(64); break;
+        case 128: LAUNCH_A(128); break;
+        case 256: LAUNCH_A(256); break;
+        case 512: LAUNCH_A(512); break;
+        default: LAUNCH_A(1024); break;
+    }
+#undef LAUNCH_A
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(y, g.ws.p, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+}  // extern "C"

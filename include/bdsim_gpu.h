@@ -1,0 +1,246 @@
+/*
+ * bdsim_gpu.h -- C ABI of the MI355X ChainIntegrator library (libbdsim_gpu.so).
+ *
+ * The library integrates batches of independent bead-spring chains with the
+ * semi-implicit predictor-corrector step of bdsim on one AMD Instinct MI355X
+ * (gfx950).  Every entry point takes plain pointers and sizes; parameter objects
+ * are plain-old-data mirrors of bdsim's C++ parameter structs, so a caller that
+ * holds a bdsim PhysParams / SimParams / Rng can fill them field by field.
+ *
+ * Reference interfaces replaced (IsaacPincus/bdsim):
+ *   bdsim_integrate_batch     <- time_integrate_chain(Vec3Field&, const PhysParams&,
+ *                                 const SimParams&, Rng&)      src/integrator.hpp:32-33
+ *                                and its binding _bdsim.integrate  bindings/module.cpp:45-50
+ *   bdsim_total_force_batch   <- _bdsim.total_force            bindings/module.cpp:67-76
+ *   bdsim_spring_force_batch  <- _bdsim.spring_force           bindings/module.cpp:52-55
+ *   bdsim_bending_force_batch <- _bdsim.bending_force          bindings/module.cpp:56-60
+ *   bdsim_ev_force_batch      <- _bdsim.ev_force               bindings/module.cpp:61-66
+ *   bdsim_external_force_batch<- _bdsim.external_force         bindings/module.cpp:128-132
+ *   bdsim_raw_noise_batch     <- raw_noise + Rng::ran_1        src/mobility.cpp:18-29,
+ *                                                              src/rng.cpp:41-70
+ *   bdsim_rpy_apply_batch     <- Diffusion::apply (D.F)        src/mobility.cpp:53-56
+ *
+ * Error convention: every entry point returns 0 on success and a negative
+ * BDSIM_E* code on an API error (bad argument, no device, out of memory); the
+ * text of the last error is available from bdsim_last_error().  Per-trajectory
+ * failures of the physics (a non positive-definite diffusion tensor, an
+ * unbracketed spring root, ...) are NOT API errors: they are reported in the
+ * status[] array, one BDSIM_STATUS_* code per trajectory, and never abort the
+ * batch.
+ */
+#ifndef BDSIM_GPU_H
+#define BDSIM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums: same integer values as bdsim's src/types.hpp:27-53 ---------- */
+enum { BDSIM_SPRING_HOOK = 1, BDSIM_SPRING_FENE = 2, BDSIM_SPRING_ILC = 3,
+       BDSIM_SPRING_WLC = 4, BDSIM_SPRING_FRAENKEL = 5,
+       BDSIM_SPRING_FENE_FRAENKEL = 6, BDSIM_SPRING_WLC_BOUNDED = 7 };
+enum { BDSIM_EV_NONE = 0, BDSIM_EV_GAUSS = 1, BDSIM_EV_LJ = 2, BDSIM_EV_SDK = 3,
+       BDSIM_EV_SDK_STICKERS = 4 };
+enum { BDSIM_BENDING_NONE = 0, BDSIM_BENDING_ONE_MINUS_COS = 1 };
+enum { BDSIM_HI_CHEBYSHEV = 0, BDSIM_HI_CHOLESKY = 1, BDSIM_HI_EXACT_SQRT = 2 };
+
+/* Precision forms of the device step. */
+enum { BDSIM_PRECISION_FP64 = 0,   /* plain FP64: parity form                */
+       BDSIM_PRECISION_MIXED = 1 };/* FP32 RPY/Chebyshev/EV pair sums with   */
+                                   /* FP64 accumulation; FP64 Lanczos bounds, */
+                                   /* Cholesky, connector solve, update      */
+
+/* Per-trajectory status codes. */
+enum {
+    BDSIM_STATUS_OK = 0,
+    BDSIM_STATUS_NOT_POSITIVE_DEFINITE = 1, /* cholesky_lower threw (linalg.cpp:16-24) */
+    BDSIM_STATUS_RTSAFE_UNBRACKETED = 2,    /* rtsafe abort (numerics.hpp:33-36)     */
+    BDSIM_STATUS_RTSAFE_MAXIT = 3,          /* rtsafe abort (numerics.hpp:71-72)     */
+    BDSIM_STATUS_EXACT_SQRT = 4,            /* ExactSqrt abort (mobility.cpp:83-85)  */
+    BDSIM_STATUS_WLCB_SIGMA = 5,            /* sigma/L > 1 abort (spring.cpp:101-105)*/
+    BDSIM_STATUS_NOT_RUN = 6                /* batch entry was skipped               */
+};
+
+/* API error codes (negative return values). */
+enum { BDSIM_OK = 0, BDSIM_EINVAL = -1, BDSIM_ENODEV = -2, BDSIM_ENOMEM = -3,
+       BDSIM_EHIP = -4 };
+
+/* ---- parameter PODs ---------------------------------------------------- */
+
+/* Flow (src/flow.hpp:17-43): constant kappa, or a (time, kappa) table that is
+ * linearly interpolated and clamped at its ends. kappa is row-major 3x3.     */
+typedef struct bdsim_flow {
+    int32_t constant;          /* nonzero: kappa0 at every t                   */
+    int32_t n_times;           /* table length (ignored when constant)         */
+    double kappa0[9];
+    const double* times;       /* [n_times], strictly ascending                */
+    const double* kappas;      /* [n_times * 9]                                */
+} bdsim_flow;
+
+/* One applied bead force (src/external_force.hpp:24-30). */
+typedef struct bdsim_bead_force {
+    int32_t bead;              /* negative counts from the end (-1 = last)     */
+    int32_t constant;
+    double value[3];
+    int32_t n_times;
+    const double* times;       /* [n_times]                                    */
+    const double* values;      /* [n_times * 3]                                */
+} bdsim_bead_force;
+
+/* PhysParams (src/model.hpp:20-35) with SpringParams (spring.hpp:12-19),
+ * EVParams (excluded_volume.hpp:17-25) and BendingParams (bending.hpp:9-12)
+ * flattened in. number_of_beads is overwritten by the chain length, as the
+ * binding does (bindings/module.cpp:47).                                     */
+typedef struct bdsim_phys {
+    int32_t spring_type;
+    double sqrtb;
+    double natural_length;
+
+    int32_t ev_type;
+    double zstar, dstar, min_cutoff, max_cutoff;
+    int32_t contour_dist_for_ev;
+    int32_t n_phi;             /* 0, or N*N (SDK attractive strengths)         */
+    const double* phi;         /* row-major [N*N], read for pairs mu<nu         */
+
+    int32_t bend_type;
+    double bend_stiffness;
+
+    bdsim_flow flow;
+    int32_t n_external;
+    const bdsim_bead_force* external;
+
+    int32_t number_of_beads;
+    double hstar;
+    int32_t hi_method;
+    double ncheb_multiplier;
+    double fd_err_max;
+    int32_t equilibration;
+} bdsim_phys;
+
+/* SimParams (src/integrator.hpp:22-28). */
+typedef struct bdsim_sim {
+    double time_start, time_end, dt, implicit_loop_tol;
+    int32_t update_center_of_mass;
+} bdsim_sim;
+
+/* Rng state (ix, iy, seed) exactly as Rng::State (src/rng.hpp:46-48). */
+typedef struct bdsim_rng_state { int32_t ix, iy, seed; } bdsim_rng_state;
+
+/* Per-trajectory work counters of one integrate call. matvecs counts every
+ * 3N x 3N mobility product D.x (Lanczos, D.x0, Chebyshev terms, predictor,
+ * upsilon, corrector); it is the n_mv of the roofline accounting.           */
+typedef struct bdsim_counters {
+    int64_t steps;
+    int64_t matvecs;
+    int64_t lanczos_iters;     /* Lanczos mat-vecs                              */
+    int64_t lanczos_cold;      /* cold Lanczos runs (incl. warm redone cold)    */
+    int64_t lanczos_warm;
+    int64_t cheb_terms;        /* sum over all FD attempts of L                 */
+    int64_t cheb_attempts;
+    int64_t fallbacks;         /* Chebyshev -> Cholesky fallbacks               */
+    int64_t corrector_iters;   /* corrector passes (each one D.x)               */
+    int64_t nonconverged;      /* corrector ran out of iterations               */
+    int32_t fail_index;        /* pivot index for NOT_POSITIVE_DEFINITE         */
+    int32_t pad_;
+    double fail_value;         /* pivot value for NOT_POSITIVE_DEFINITE         */
+} bdsim_counters;
+
+/* Optional per-step decision log (one entry per step of one trajectory). */
+typedef struct bdsim_step_log {
+    double t;
+    double lmin, lmax;         /* Chebyshev interval (0 when not Chebyshev)    */
+    double fd_err;             /* final FD error                                */
+    int32_t lanczos_cold;      /* 1 if this step's Lanczos ran cold             */
+    int32_t lanczos_steps;     /* Krylov dimension of the accepted run          */
+    int32_t cheb_L;            /* final term count                              */
+    int32_t cheb_attempts;
+    int32_t fallback;
+    int32_t corrector_iters;   /* corrector passes this step                    */
+} bdsim_step_log;
+
+/* ---- library lifetime -------------------------------------------------- */
+
+/* Selects the device and uploads constant tables. Idempotent. */
+int bdsim_init(int device);
+/* Text of the last API error of the calling thread ("" if none). */
+const char* bdsim_last_error(void);
+/* Largest supported chain length (beads per trajectory). */
+int bdsim_max_beads(void);
+
+/* ---- the step: batched time_integrate_chain ---------------------------- */
+
+/* Integrates n_traj independent chains, each over its own segment
+ * [sim[i].time_start, sim[i].time_end] exactly as time_integrate_chain does
+ * (fresh spectral cache per call, accumulated t += dt clock).
+ *   nbeads[i]   chain length of trajectory i (2 .. bdsim_max_beads())
+ *   R           host, packed: trajectory i occupies 3*nbeads[i] doubles
+ *               starting at 3*offset(i), offset = prefix sum of nbeads;
+ *               positions in on entry, integrated positions out
+ *   phys, sim   host arrays [n_traj]
+ *   rng         host [n_traj], advanced in place like Rng& is
+ *   precision   BDSIM_PRECISION_FP64 or BDSIM_PRECISION_MIXED
+ *   status      host [n_traj] out, BDSIM_STATUS_*
+ *   counters    host [n_traj] out, or NULL
+ *   log         host, or NULL; trajectory i writes its steps at
+ *               log[i*log_stride ...], at most log_stride entries
+ * Blocks until the results are back on the host.                            */
+int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R,
+                          const bdsim_phys* phys, const bdsim_sim* sim,
+                          bdsim_rng_state* rng, int precision,
+                          int32_t* status, bdsim_counters* counters,
+                          bdsim_step_log* log, int log_stride);
+
+/* ---- device-resident batches (for the serving loop and benchmarks) ----- */
+
+/* An opaque batch whose positions, RNG states and workspace stay in HBM
+ * between calls. All chains in one batch share (phys, sim) except the flow,
+ * which may differ per trajectory (per_traj_kappa), and the RNG seed.        */
+typedef struct bdsim_batch bdsim_batch;
+
+/* Creates a device-resident batch. R0 host [n_traj*N*3], rng host [n_traj],
+ * kappa host [n_traj*9] or NULL (then phys->flow applies to all).           */
+int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads,
+                       const double* R0, const bdsim_phys* phys,
+                       const double* kappa, const bdsim_rng_state* rng,
+                       int precision);
+/* Runs one integrate segment [sim->time_start, sim->time_end] on every
+ * trajectory of the batch on the batch's stream; does not synchronise.     */
+int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim);
+/* Waits for the batch's stream. */
+int bdsim_batch_sync(bdsim_batch* b);
+/* Copies positions / rng / status / counters back (any may be NULL). */
+int bdsim_batch_fetch(bdsim_batch* b, double* R, bdsim_rng_state* rng,
+                      int32_t* status, bdsim_counters* counters);
+/* Device-side HIP stream handle (hipStream_t) of the batch, for timing. */
+void* bdsim_batch_stream(bdsim_batch* b);
+void bdsim_batch_destroy(bdsim_batch* b);
+
+/* ---- forces and pieces, batched over chains of one length -------------- */
+
+/* R host [n_traj*N*3] -> F host [n_traj*N*3]. */
+int bdsim_total_force_batch(int n_traj, int nbeads, const double* R,
+                            const bdsim_phys* phys, double* F);
+int bdsim_spring_force_batch(int n_traj, int nbeads, const double* R,
+                             const bdsim_phys* phys, double* F);
+int bdsim_bending_force_batch(int n_traj, int nbeads, const double* R,
+                              const bdsim_phys* phys, double* F);
+int bdsim_ev_force_batch(int n_traj, int nbeads, const double* R,
+                         const bdsim_phys* phys, double* F);
+/* External force of phys on nbeads beads at time t -> F host [nbeads*3]. */
+int bdsim_external_force_batch(const bdsim_phys* phys, int nbeads, double t,
+                               double* F);
+/* n_steps consecutive raw_noise fields per trajectory:
+ * rng host [n_traj] in/out, X host [n_traj*n_steps*nbeads*3].               */
+int bdsim_raw_noise_batch(int n_traj, int nbeads, int n_steps,
+                          bdsim_rng_state* rng, double* X);
+/* y = D(R).x for each chain (RPY with hstar; identity when hstar <= 0). */
+int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision,
+                          const double* R, const double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BDSIM_GPU_H */

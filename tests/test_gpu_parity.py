@@ -1,0 +1,288 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle and
+the golden fixtures written from the reference core.
+
+Bars: the RNG stream is bit-exact; forces within 1e-12 relative (the
+reference's own check is 1e-6); D.x within 1e-13 (FP64) / 1e-5 (mixed);
+FP64 positions within 1e-10 relative of the reference over 100 steps with
+identical per-step decisions (cold/warm Lanczos, Chebyshev terms and attempts,
+fallbacks, corrector passes); the mixed form within 1e-5 over 100 steps.
+"""
+import ctypes
+from ctypes import byref
+
+import numpy as np
+import pytest
+
+import helpers as H
+from bdsim_amd import _abi as A
+from golden import cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+# --------------------------------------------------------------------------
+# RNG
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("N", [1, 2, 10, 50, 64, 65, 129, 300, 1000, 1024])
+def test_noise_bit_exact(gpu, orc, N):
+    seeds = [0, 5, -3, 4095, 123456789]
+    n_steps = 3
+    st = (A.CRng * len(seeds))(*[H.rng_of(s) for s in seeds])
+    X = np.empty(len(seeds) * n_steps * 3 * N)
+    A.check(gpu, gpu.bdsim_raw_noise_batch(len(seeds), N, n_steps, st, A.dptr(X)))
+    X = X.reshape(len(seeds), n_steps, 3 * N)
+    for k, s in enumerate(seeds):
+        so = H.rng_of(s)
+        for t in range(n_steps):
+            xo = np.empty(3 * N)
+            orc.bo_raw_noise(byref(so), N, A.dptr(xo))
+            assert np.array_equal(X[k, t], xo), (N, s, t)
+        assert (st[k].ix, st[k].iy, st[k].seed) == (so.ix, so.iy, so.seed)
+
+
+def test_noise_matches_golden_uniforms(gpu, golden):
+    """Invert the Ottinger transform's input: draws 3*mu..3*mu+2 of step 0 are
+    the golden uniforms (for seeds whose fixtures cover 3*N draws)."""
+    N = 600
+    for k, s in enumerate(golden["rng_seeds"]):
+        st = (A.CRng * 1)(H.rng_of(int(s)))
+        X = np.empty(3 * N)
+        A.check(gpu, gpu.bdsim_raw_noise_batch(1, N, 1, st, A.dptr(X)))
+        y = golden["rng_draws"][k][:3 * N].astype(np.float64) - 0.5
+        assert np.array_equal(X, (14.14855378 * y * y + 1.21569221) * y)
+
+
+# --------------------------------------------------------------------------
+# forces and mobility
+# --------------------------------------------------------------------------
+
+def test_forces_match_golden(gpu, golden):
+    import json
+    names = sorted({k.split("/")[1] for k in golden.files if k.startswith("force/")})
+    fn = {"spring": "bdsim_spring_force_batch", "ev": "bdsim_ev_force_batch",
+          "bend": "bdsim_bending_force_batch", "total": "bdsim_total_force_batch"}
+    for nm in names:
+        d = json.loads(str(golden[f"force/{nm}/phys"]))
+        if f"force/{nm}/phi" in golden.files:
+            d["phi"] = golden[f"force/{nm}/phi"]
+        R = np.ascontiguousarray(golden[f"force/{nm}/R"])
+        p = H.Packed(d, R.shape[0])
+        F = np.empty_like(R)
+        A.check(gpu, getattr(gpu, fn[str(golden[f"force/{nm}/kind"])])(
+            1, R.shape[0], A.dptr(R), byref(p.c), A.dptr(F)))
+        want = golden[f"force/{nm}/F"]
+        assert rel(F, want) < 1e-12, nm
+
+
+def test_external_force_and_flow_tables(gpu, golden):
+    d = C.phys_dict(external=((0, True, (0.0, 0.0, -3.0), (), ()),
+                              (-1, False, (0, 0, 0), (0.0, 0.5, 1.0),
+                               ((0.0, 0.0, 0.0), (0.0, 0.0, 6.0), (1.0, 2.0, 3.0))),
+                              (3, True, (1.0, -2.0, 0.5), (), ()),
+                              (3, True, (0.5, 0.5, 0.5), (), ()),
+                              (17, True, (9.0, 9.0, 9.0), (), ())))
+    p = H.Packed(d, 6)
+    for i, t in enumerate(golden["external/t"]):
+        F = np.empty((6, 3))
+        A.check(gpu, gpu.bdsim_external_force_batch(byref(p.c), 6, float(t), A.dptr(F)))
+        assert np.array_equal(F, golden["external/F"][i])
+
+
+@pytest.mark.parametrize("N,h", [(2, 0.3), (10, 0.3), (50, 0.3), (64, 0.2), (100, 0.25), (300, 0.3)])
+def test_rpy_apply(gpu, orc, N, h):
+    R = np.ascontiguousarray(C.random_walk(N, 40 + N, 0.8))
+    x = np.ascontiguousarray(np.random.default_rng(N).normal(size=(N, 3)))
+    D = np.empty((3 * N, 3 * N))
+    orc.bo_rpy_matrix(N, A.dptr(R), h, A.dptr(D))
+    want = D @ x.ravel()
+    for prec, tol in ((A.PRECISION_FP64, 1e-13), (A.PRECISION_MIXED, 1e-5)):
+        y = np.empty_like(x)
+        A.check(gpu, gpu.bdsim_rpy_apply_batch(1, N, h, prec, A.dptr(R), A.dptr(x), A.dptr(y)))
+        assert rel(y.ravel(), want) < tol, (prec, rel(y.ravel(), want))
+
+
+# --------------------------------------------------------------------------
+# the integrator
+# --------------------------------------------------------------------------
+
+def _golden_items(golden):
+    cases = {c["name"]: c for c in C.trajectory_cases()}
+    return [(cases[str(n)], golden) for n in golden["traj_names"]]
+
+
+def test_trajectories_fp64_parity(gpu, orc, golden):
+    """Every golden trajectory, all chains in ONE batched launch per segment:
+    positions <= 1e-10 relative of the reference, identical RNG state and the
+    same discrete decisions as the oracle, step by step."""
+    cases = [c for c, _ in _golden_items(golden)]
+    keep = [H.Packed(c["phys"], c["N"]) for c in cases]
+    Rs = [golden[f"traj/{c['name']}/R0"] for c in cases]
+    sts = [H.rng_of(c["seed"]) for c in cases]
+    oracle_sts = [H.rng_of(c["seed"]) for c in cases]
+    n_seg = max(c["segments"] for c in cases)
+    worst = 0.0
+    for seg in range(n_seg):
+        idx = [i for i, c in enumerate(cases) if seg < c["segments"]]
+        items = [(cases[i]["N"], Rs[i], keep[i].c, H.csim(cases[i]["sim"]), sts[i]) for i in idx]
+        out, rngs, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=200)
+        for j, i in enumerate(idx):
+            c = cases[i]
+            name = c["name"]
+            assert status[j] == golden[f"traj/{name}/rc"][seg], name
+            sts[i] = rngs[j]
+            Ro, rco, cnto, lo = H.oracle_integrate(orc, c["N"], Rs[i], keep[i].c,
+                                                   H.csim(c["sim"]), oracle_sts[i], log_cap=200)
+            assert rco == status[j]
+            if status[j] != 0:
+                continue
+            want = golden[f"traj/{name}/R"][seg]
+            r = rel(out[j], want)
+            worst = max(worst, r)
+            assert r <= 1e-10, (name, seg, r)
+            lg = [logs[j * 200 + k] for k in range(cnts[j].steps)]
+            assert len(lg) == len(lo) == cnto.steps
+            for k, (a, b) in enumerate(zip(lg, lo)):
+                assert (a.lanczos_cold, a.lanczos_steps, a.cheb_L, a.cheb_attempts, a.fallback,
+                        a.corrector_iters) == (b.lanczos_cold, b.lanczos_steps, b.cheb_L,
+                                               b.cheb_attempts, b.fallback, b.corrector_iters), (name, k)
+                assert a.t == b.t
+            Rs[i] = out[j]
+    for c, st in zip(cases, sts):
+        assert (st.ix, st.iy, st.seed) == tuple(golden[f"traj/{c['name']}/rng"]), c["name"]
+    print(f"worst relative position error over all golden trajectories: {worst:.3e}")
+
+
+def test_rouse_kat_on_gpu(gpu):
+    # tests/test_integrator_rouse.cpp:49-94
+    p = H.Packed(C.phys_dict(spring=1, sqrtb=1000.0), 10)
+    st = H.rng_of(5)
+    R = C.R0_ROUSE
+    for ans in (C.ROUSE_ANS1, C.ROUSE_ANS2):
+        out, rngs, status, _, _ = H.gpu_batch(gpu, [(10, R, p.c, H.csim(C.sim_dict(0.0, 1.0, 0.1, 1e-6)), st)])
+        st = rngs[0]
+        assert np.max(np.abs(out[0] - ans)) < 1e-6
+        R = out[0]
+
+
+def test_singular_configuration_fails_cleanly(gpu, golden):
+    p = H.Packed(C.phys_dict(spring=1, sqrtb=1000.0, hstar=C.SINGULAR_HSTAR, method=1), 21)
+    ok = H.Packed(C.phys_dict(spring=1, sqrtb=1000.0, hstar=0.2, method=1), 21)
+    sim = H.csim(C.sim_dict(0.0, 0.0, 0.01, 1e-6))
+    out, rngs, status, cnts, _ = H.gpu_batch(gpu, [(21, C.R0_SINGULAR, p.c, sim, H.rng_of(6)),
+                                                   (21, C.random_walk(21, 3), ok.c, sim, H.rng_of(7))])
+    assert status[0] == A.STATUS_NOT_POSITIVE_DEFINITE and cnts[0].fail_index == 55
+    assert (rngs[0].ix, rngs[0].iy, rngs[0].seed) == tuple(golden["singular_integrate/rng"])
+    assert status[1] == 0 and np.all(np.isfinite(out[1]))   # the batch went on
+
+
+def test_uneven_batch_equals_single_runs(gpu):
+    """Chains of different N, springs, EV, flows in one call give the same
+    result, bit for bit, as each alone (no cross-talk between workgroups)."""
+    specs = []
+    rs = np.random.default_rng(5)
+    for k, N in enumerate([10, 33, 64, 65, 130, 12, 257, 50]):
+        d = C.phys_dict(spring=[2, 6, 1, 3][k % 4], sqrtb=10.0, nat=2.0 if k % 4 == 1 else 0.0,
+                        hstar=[0.3, 0.0, 0.25, 0.2][k % 4], ev=[3, 1, 0, 2][k % 4], zstar=0.5,
+                        dstar=1.0, kappa=C.wi_kappa(float(rs.uniform(0.1, 10))), method=k % 2)
+        specs.append((N, C.random_walk(N, 600 + k), H.Packed(d, N),
+                      H.csim(C.sim_dict(0.0, 0.019, 1e-3, 1e-3)), H.rng_of(k)))
+    items = [(N, R, p.c, s, st) for N, R, p, s, st in specs]
+    together, rt, stt, _, _ = H.gpu_batch(gpu, items)
+    for j, it in enumerate(items):
+        alone, ra, sa, _, _ = H.gpu_batch(gpu, [it])
+        assert np.array_equal(alone[0], together[j]) and sa[0] == stt[j]
+        assert (ra[0].ix, ra[0].iy) == (rt[j].ix, rt[j].iy)
+
+
+def test_mixed_precision_within_1e5(gpu, orc):
+    """Headline physics (N=50, FENE b=100, h*=0.3 Chebyshev, SDK, shear):
+    the mixed form stays within 1e-5 relative of the FP64 oracle over 100 steps."""
+    items, ref = [], []
+    for i, wi in enumerate([0.1, 1.0, 10.0, 100.0]):
+        d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.3, ev=3, dstar=1.0, kappa=C.wi_kappa(wi))
+        p = H.Packed(d, 50)
+        R0 = C.random_walk(50, 1000 + i)
+        sim = H.csim(C.sim_dict(0.0, 0.099, 1e-3, 1e-3))
+        items.append((50, R0, p.c, sim, H.rng_of(i)))
+        ref.append(H.oracle_integrate(orc, 50, R0, p.c, sim, H.rng_of(i))[0])
+        items[-1] = items[-1] + (p,)
+    out, _, status, _, _ = H.gpu_batch(gpu, [it[:5] for it in items], precision=A.PRECISION_MIXED)
+    assert np.all(status == 0)
+    for o, r in zip(out, ref):
+        assert rel(o, r) < 1e-5, rel(o, r)
+
+
+def test_device_resident_batch_matches(gpu):
+    """bdsim_batch_* (positions and RNG resident in HBM) reproduces
+    bdsim_integrate_batch over two consecutive segments."""
+    n, N = 6, 20
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.3, ev=3, dstar=1.0)
+    p = H.Packed(d, N)
+    R0 = np.ascontiguousarray(np.array([C.random_walk(N, 700 + i) for i in range(n)]))
+    kap = np.ascontiguousarray(np.array([C.wi_kappa(0.5 * i) for i in range(n)]))
+    rng = (A.CRng * n)(*[H.rng_of(i) for i in range(n)])
+    b = ctypes.c_void_p()
+    A.check(gpu, gpu.bdsim_batch_create(byref(b), n, N, A.dptr(R0), byref(p.c), A.dptr(kap), rng,
+                                        A.PRECISION_FP64))
+    sim = A.CSim(0.0, 0.009, 1e-3, 1e-3, 1)
+    try:
+        A.check(gpu, gpu.bdsim_batch_run(b, byref(sim)))
+        A.check(gpu, gpu.bdsim_batch_run(b, byref(sim)))
+        Rb = np.empty_like(R0)
+        st = (A.CRng * n)()
+        status = np.zeros(n, dtype=np.int32)
+        A.check(gpu, gpu.bdsim_batch_fetch(b, A.dptr(Rb), st, A.iptr(status), None))
+    finally:
+        gpu.bdsim_batch_destroy(b)
+    for i in range(n):
+        di = dict(d, kappa=C.wi_kappa(0.5 * i))
+        pi = H.Packed(di, N)
+        s = H.rng_of(i)
+        R = R0[i]
+        for _ in range(2):
+            out, rr, stt, _, _ = H.gpu_batch(gpu, [(N, R, pi.c, sim, s)])
+            R, s = out[0], rr[0]
+        assert np.array_equal(R, Rb[i]) and (s.ix, s.iy, s.seed) == (st[i].ix, st[i].iy, st[i].seed)
+
+
+def test_core_api_integrate(gpu, orc):
+    """The `_bdsim`-compatible Python surface drives the GPU and matches the oracle."""
+    import pickle
+    import bdsim_amd as bd
+    phys = bd.PhysParams()
+    phys.spring.type = bd.Spring.FENEFraenkel
+    phys.spring.sqrtb = 2.0
+    phys.spring.natural_length = 1.0
+    phys.hstar = 0.3
+    phys.flow = bd.Flow.constant(np.array([[0, 1.0, 0], [0, 0, 0], [0, 0, 0]]))
+    sim = bd.SimParams()
+    sim.time_start, sim.time_end, sim.dt = 0.0, 0.99, 0.01
+    rng = bd.Rng(3)
+    R0 = C.ff_chain(10, 103)
+    phys2 = pickle.loads(pickle.dumps(phys))
+    R = bd.integrate(R0, phys2, sim, rng)
+    assert phys2.number_of_beads == 10   # the caller's object is untouched
+    d = C.phys_dict(spring=6, sqrtb=2.0, nat=1.0, hstar=0.3, kappa=C.SHEAR)
+    p = H.Packed(d, 10)
+    st = H.rng_of(3)
+    Ro, _, _, _ = H.oracle_integrate(orc, 10, R0, p.c, H.csim(C.sim_dict(0.0, 0.99, 0.01, 1e-3)), st)
+    assert rel(R, Ro) < 1e-10
+    assert rng.__getstate__() == (st.ix, st.iy, st.seed)
+    with pytest.raises(RuntimeError, match="pivot 55"):
+        ph = bd.PhysParams()
+        ph.spring.sqrtb = 1000.0
+        ph.hstar = C.SINGULAR_HSTAR
+        ph.hi_method = bd.DelSMethod.Cholesky
+        s1 = bd.SimParams()
+        s1.time_end = 0.0
+        bd.integrate(C.R0_SINGULAR, ph, s1, bd.Rng(6))
+    Ft = bd.total_force(C.R0_FULL, phys)
+    Fo = np.empty_like(C.R0_FULL)
+    orc.bo_total_force(15, A.dptr(np.ascontiguousarray(C.R0_FULL)), byref(H.Packed(
+        dict(d, phi=None), 15).c), A.dptr(Fo))
+    assert rel(Ft, Fo) < 1e-12
