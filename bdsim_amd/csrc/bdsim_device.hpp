@@ -66,9 +66,25 @@ struct RngState { int32_t ix, iy, seed; };
 enum : int32_t { ST_OK = 0, ST_NOT_PD = 1, ST_RTSAFE_UNBRACKETED = 2, ST_RTSAFE_MAXIT = 3,
                  ST_EXACT_SQRT = 4, ST_WLCB_SIGMA = 5 };
 
-// Workspace doubles per trajectory.
-__host__ __device__ inline int64_t ws_lanczos_doubles(int N) { return (int64_t)LANCZOS_COLD * 3 * N; }
-__host__ __device__ inline int64_t ws_dense_doubles(int N) { return (int64_t)9 * N * N; }
+// Per-trajectory workspace in HBM (doubles). The state slots hold per-bead
+// vectors that live across phases of a step (structure of arrays: slot k of
+// bead mu at S + k*N + mu), so they need not occupy registers meanwhile.
+enum : int { SLOT_FS = 0, SLOT_FE = 3, SLOT_DW = 6, SLOT_SEED = 9, SLOT_X0 = 12, N_SLOTS = 16 };
+struct WsLayout {
+    int64_t S, coef, aux, tri, V, dense, total;
+};
+__host__ __device__ inline WsLayout ws_layout(int N, bool hi) {
+    WsLayout w;
+    int64_t o = 0;
+    w.S = o;      o += (int64_t)N_SLOTS * N;
+    w.coef = o;   o += MAXCHEB + 8;                       // Chebyshev coefficients
+    w.aux = o;    o += MAXCHEB + 8;                       // sqrt at the Chebyshev nodes
+    w.tri = o;    o += 2 * 3 * LANCZOS_COLD + 4;           // inverse-iteration rows
+    w.V = o;      o += hi ? (int64_t)LANCZOS_COLD * 3 * N : 0;   // Lanczos basis
+    w.dense = o;  o += hi ? (int64_t)9 * N * N : 0;              // dense D / its factor
+    w.total = (o + 31) & ~(int64_t)31;
+    return w;
+}
 
 // ---- reductions ----------------------------------------------------------
 // Butterfly sum: every lane ends with the same value (IEEE addition commutes),

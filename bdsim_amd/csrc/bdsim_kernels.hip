@@ -9,6 +9,11 @@
 // linalg.cpp:81-89 fused). The dense tensor is built only for the Cholesky
 // method and for the rare Chebyshev -> Cholesky fallback (mobility.cpp:71-80).
 //
+// Occupancy: the LDS footprint is ~9 KB for chains of up to 64 beads and the
+// per-bead vectors that live across phases of a step sit in a per-trajectory
+// HBM slot, so a 64-thread workgroup fits in 128 VGPRs and 16 chains share a
+// CU (4 waves per SIMD).
+//
 // Citations are to IsaacPincus/bdsim.
 #include "bdsim_device.hpp"
 #include "../../include/bdsim_gpu.h"
@@ -43,33 +48,25 @@ struct KArgs {
     double t0, t1, dt, tol;
 };
 
-// Mat-vec operand buffers: two (one barrier per mat-vec) while they are small,
-// one (two barriers) for the large-chain blocks, to stay inside the LDS.
-template <int BLOCK> struct XBufs { static constexpr int value = BLOCK <= 256 ? 2 : 1; };
-
 template <int BLOCK, bool MIXED>
 struct Smem {
     static constexpr int NW = BLOCK / 64;
-    static constexpr int FB = MIXED ? BLOCK : 1;
-    static constexpr int XB = XBufs<BLOCK>::value;
+    static constexpr int FB = MIXED ? BLOCK : 1;   // FP32 arrays (mixed form)
+    static constexpr int DB = MIXED ? 1 : BLOCK;   // FP64 operand (FP64 form)
     double R[3][BLOCK];         // configuration the mobility is evaluated at
-    double P[3][BLOCK];         // configuration the forces are evaluated at
-    double X[XB][3][BLOCK];     // mat-vec operand
-    double B[4][BLOCK];         // bond-level arrays (unit bonds + lengths)
-    double V[3][BLOCK];         // neighbour exchange
+    double P[3][BLOCK];         // force configuration; corrector positions
+    double X[3][DB];            // mat-vec operand
     float Rf[3][FB];            // mixed form: positions relative to bead 0
-    float Xf[XB][3][FB];        // mixed form: mat-vec operand
+    float Xf[3][FB];            // mixed form: mat-vec operand
+    double B[4][BLOCK];         // bond-level arrays (unit bonds + lengths, bonds)
+    double V[3][BLOCK];         // neighbour exchange; triangular-product operand
     double Red[2 * NW];
-    double Coef[MAXCHEB + 1];
-    // sqrt of the Chebyshev nodes; for large blocks it lives in V, which is
-    // idle while the coefficients are built
-    double Aux[BLOCK >= 256 ? 1 : MAXCHEB + 1];
-    __device__ double* aux() { return BLOCK >= 256 ? &V[0][0] : Aux; }
     double Alpha[LANCZOS_COLD];
     double Beta[LANCZOS_COLD];
-    double Tri[2][3][LANCZOS_COLD];
     double Scal[16];
     int32_t ScalI[8];
+    Counters cnt;               // work counters, updated by thread 0
+    StepLog lg;                 // this step's decisions, written by thread 0
 };
 
 __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
@@ -89,7 +86,7 @@ __device__ inline int sturm_count(int k, const double* a, const double* b, doubl
     return count;
 }
 
-// hydrodynamics.cpp:106-136; c, dd, y are LDS rows of this lane.
+// hydrodynamics.cpp:106-136; c, dd, y are this lane's rows of the workspace.
 __device__ inline void tridiag_eigenvector(int m, const double* a, const double* b, double lambda,
                                            double* c, double* dd, double* y) {
     const double scale = (fabs(lambda) < 1.0) ? 1.0 : fabs(lambda);
@@ -119,34 +116,80 @@ __device__ inline void tridiag_eigenvector(int m, const double* a, const double*
     }
 }
 
+// Extreme Ritz value of the k x k tridiagonal by 100 Sturm bisections from the
+// Gershgorin bracket (hydrodynamics.cpp:240-262), and its eigenvector; lane 0
+// finds the smallest (count >= 1), lane 1 the largest (count >= k).
+__device__ __noinline__ double ritz_extreme(int k, const double* al, const double* be, int lane,
+                                            double* tri) {
+    double lo = al[0], hi = al[0];
+    for (int i = 0; i < k; ++i) {
+        const double rad = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < k - 1 ? fabs(be[i]) : 0.0);
+        if (al[i] - rad < lo) lo = al[i] - rad;
+        if (hi < al[i] + rad) hi = al[i] + rad;
+    }
+    const int target = lane == 0 ? 1 : k;
+    double a1 = lo, b1 = hi;
+    for (int it = 0; it < 100; ++it) {
+        const double mid = 0.5 * (a1 + b1);
+        if (sturm_count(k, al, be, mid) >= target) b1 = mid; else a1 = mid;
+    }
+    const double r = 0.5 * (a1 + b1);
+    double* rows = tri + lane * 3 * LANCZOS_COLD;
+    tridiag_eigenvector(k, al, be, r, rows, rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
+    return r;
+}
+
+// spring.cpp:174-181 out of line: the cubic / rtsafe solvers are large and run
+// once per bond per corrector pass.
+__device__ __noinline__ int solve_connector_ool(double g0, double g1, double g2, double dt,
+                                                int type, double sqrtb, double q0, double* out) {
+    const double g[3] = {g0, g1, g2};
+    double o[3];
+    const int st = solve_connector(g, dt, type, sqrtb, q0, o);
+    out[0] = o[0]; out[1] = o[1]; out[2] = o[2];
+    return st;
+}
+
 template <int BLOCK, bool MIXED>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
     Smem<BLOCK, MIXED>& sm;
-    const TrajParams& P;
+    const TrajParams& P;        // global, read with scalar loads
     const KArgs& A;
     const int N, tid, n3;
     const bool own;
     const bool hi;
-    const RpyConsts rc;
-    double* Vb;                 // Lanczos basis [LANCZOS_COLD][3N]
-    double* Dm;                 // dense mobility / Cholesky factor [3N][3N]
-    int par = 0, xpar = 0;
-    Counters cnt;
+    const WsLayout wl;
+    double* ws;                 // this trajectory's workspace
+    int par = 0;
 
     __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
-          hi(p.hstar > 0.0), rc(rpy_consts(p.hstar)) {
-        Vb = A.ws + P.ws_off;
-        Dm = Vb + ws_lanczos_doubles(N);
-        cnt = Counters{};
+          hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
+        ws = A.ws + P.ws_off;
     }
 
     __device__ __forceinline__ double bsum(double v) { return block_sum<NW>(v, sm.Red, par); }
+    __device__ __forceinline__ void bump(int64_t& c, int64_t v = 1) { if (tid == 0) c += v; }
+    __device__ __forceinline__ void bump32(int32_t& c) { if (tid == 0) c += 1; }
+    template <typename T, typename U>
+    __device__ __forceinline__ void put(T& dst, U v) { if (tid == 0) dst = (T)v; }
 
-    // Stage the mobility configuration (sm.R) and, for the mixed form, its
-    // FP32 copy relative to bead 0. Caller has written sm.R and synchronised.
-    __device__ void stage_mixed() {
+    // per-bead state slots in HBM (own bead only; no synchronisation needed)
+    __device__ __forceinline__ void save(int slot, const double v[3]) {
+        if (own)
+            for (int d = 0; d < 3; ++d) ws[wl.S + (int64_t)(slot + d) * N + tid] = v[d];
+    }
+    __device__ __forceinline__ void load(int slot, double v[3]) {
+        for (int d = 0; d < 3; ++d) v[d] = own ? ws[wl.S + (int64_t)(slot + d) * N + tid] : 0.0;
+    }
+
+    // Stage the mobility configuration: sm.R from the caller's R, and for the
+    // mixed form its FP32 copy relative to bead 0.
+    __device__ void stage_R(const double R[3]) {
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
+        __syncthreads();
         if constexpr (MIXED) {
             for (int d = 0; d < 3; ++d)
                 sm.Rf[d][tid] = own ? (float)(sm.R[d][tid] - sm.R[d][0]) : 0.0f;
@@ -157,41 +200,55 @@ struct Chain {
     // y = D(sm.R) v   (Diffusion::apply, mobility.cpp:53-56)
     __device__ void apply(const double v[3], double y[3]) {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
-        constexpr int XB = XBufs<BLOCK>::value;
-        const int b = XB == 2 ? xpar : 0;
-        xpar ^= 1;
-        if constexpr (XB == 1) __syncthreads();
-        for (int d = 0; d < 3; ++d) sm.X[b][d][tid] = v[d];
-        if constexpr (MIXED)
-            for (int d = 0; d < 3; ++d) sm.Xf[b][d][tid] = (float)v[d];
         __syncthreads();
-        ++cnt.matvecs;
+        if constexpr (MIXED) {
+            for (int d = 0; d < 3; ++d) sm.Xf[d][tid] = (float)v[d];
+        } else {
+            for (int d = 0; d < 3; ++d) sm.X[d][tid] = v[d];
+        }
+        __syncthreads();
+        bump(sm.cnt.matvecs);
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (own) {
             if constexpr (!MIXED) {
+                const double hs = P.hstar, ihs = 1.0 / P.hstar;
+                const double RPI = 1.7724538509055160273;
+                const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0,
+                             C3 = 0.75 / RPI * 0.375, C7 = 0.09375 / RPI, R34 = RPI * 0.75;
                 const double rx = sm.R[0][tid], ry = sm.R[1][tid], rz = sm.R[2][tid];
+#pragma unroll 2
                 for (int nu = 0; nu < N; ++nu) {
                     const double bx = sm.R[0][nu] - rx, by = sm.R[1][nu] - ry, bz = sm.R[2][nu] - rz;
-                    double om1, om2;
-                    rpy_weights64(rc, bx * bx + by * by + bz * bz, om1, om2);
+                    double sq = bx * bx + by * by + bz * bz;
+                    sq = sq < 1.0e-12 ? 1.0e-12 : sq;
+                    // RPY pair weights (hydrodynamics.cpp:88-98) from one rsqrt
+                    const double inv_rs = rsqrt(sq);
+                    const double rs = sq * inv_rs;
+                    const double hsbyrs = hs * inv_rs, rsbyhs = rs * ihs;
+                    const double hs2 = hsbyrs * hsbyrs, ir2 = inv_rs * inv_rs;
+                    const bool far = rsbyhs >= TRPI;
+                    double om1 = far ? R34 * hsbyrs * (1.0 + C1 * hs2) : 1.0 - C3 * rsbyhs;
+                    double om2 = far ? R34 * (hsbyrs * ir2) * (1.0 - TPI * hs2) : C7 * rsbyhs * ir2;
                     if (nu == tid) { om1 = 1.0; om2 = 0.0; }
-                    const double xx = sm.X[b][0][nu], xy = sm.X[b][1][nu], xz = sm.X[b][2][nu];
+                    const double xx = sm.X[0][nu], xy = sm.X[1][nu], xz = sm.X[2][nu];
                     const double t = om2 * (bx * xx + by * xy + bz * xz);
                     ax += om1 * xx + t * bx;
                     ay += om1 * xy + t * by;
                     az += om1 * xz + t * bz;
                 }
             } else {
-                const float hs = (float)rc.hstar, ihs = (float)rc.inv_hstar, TRPI = (float)rc.TRPI,
-                            TPI = (float)rc.TPI, C1 = (float)rc.C1, C3 = (float)rc.C3,
-                            C7 = (float)rc.C7, R34 = (float)rc.RPI3by4;
+                const float hs = (float)P.hstar, ihs = (float)(1.0 / P.hstar);
+                const float RPI = 1.7724538509055160273f;
+                const float TRPI = 2.0f * RPI, TPI = (float)(2.0 * PI), C1 = (float)(2.0 * PI / 3.0),
+                            C3 = 0.75f / RPI * 0.375f, C7 = 0.09375f / RPI, R34 = RPI * 0.75f;
                 const float rx = sm.Rf[0][tid], ry = sm.Rf[1][tid], rz = sm.Rf[2][tid];
                 int nu = 0;
                 while (nu < N) {
-                    // FP32 pair terms summed over a run of 8 partners, then
+                    // FP32 pair terms summed over runs of 8 partners, then
                     // accumulated in FP64
                     const int end = min(nu + 8, N);
                     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll 2
                     for (; nu < end; ++nu) {
                         const float bx = sm.Rf[0][nu] - rx, by = sm.Rf[1][nu] - ry,
                                     bz = sm.Rf[2][nu] - rz;
@@ -199,7 +256,7 @@ struct Chain {
                         rpy_weights32(hs, ihs, TRPI, TPI, C1, C3, C7, R34,
                                       bx * bx + by * by + bz * bz, om1, om2);
                         if (nu == tid) { om1 = 1.0f; om2 = 0.0f; }
-                        const float xx = sm.Xf[b][0][nu], xy = sm.Xf[b][1][nu], xz = sm.Xf[b][2][nu];
+                        const float xx = sm.Xf[0][nu], xy = sm.Xf[1][nu], xz = sm.Xf[2][nu];
                         const float t = om2 * (bx * xx + by * xy + bz * xz);
                         fx += om1 * xx + t * bx;
                         fy += om1 * xy + t * by;
@@ -231,15 +288,15 @@ struct Chain {
         }
     }
 
-    // Bending force (bending.cpp:5-43) at the configuration in sm.P.
-    __device__ void bending_force(double F[3]) {
+    // Bending force (bending.cpp:5-43) at configuration C (LDS, [3][BLOCK]).
+    __device__ void bending_force(const double (*C)[BLOCK], double F[3]) {
         F[0] = F[1] = F[2] = 0.0;
         if (P.bend_type == 0 || N == 2) return;
-        const double C = P.bend_C;
+        const double Cst = P.bend_C;
         __syncthreads();
         if (tid < N - 1) {
             double b[3];
-            for (int d = 0; d < 3; ++d) b[d] = sm.P[d][tid + 1] - sm.P[d][tid];
+            for (int d = 0; d < 3; ++d) b[d] = C[d][tid + 1] - C[d][tid];
             double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
             if (sq < 1.0e-12) sq = 1.0e-12;
             const double len = sqrt(sq);
@@ -251,8 +308,8 @@ struct Chain {
         const int m = tid;
         auto u = [&](int k, int d) { return sm.B[d][k]; };
         auto len = [&](int k) { return sm.B[3][k]; };
-        // bend cosines cs(c) = u_{c+1} . u_c (config.cpp:41-45) for the three
-        // angles this bead takes part in; index c is shifted by 2 below.
+        // bend cosines cs[c] = u_{c+1} . u_c (config.cpp:41-45) of the three
+        // angles this bead takes part in
         double csl[3];
         for (int c = m - 2, i = 0; i < 3; ++c, ++i)
             csl[i] = (c >= 0 && c < N - 2)
@@ -266,47 +323,48 @@ struct Chain {
         auto term2 = [&](double s, double ca, int a, int bb, double* out) {  // s*((-ca)*u_a + u_b)
             for (int d = 0; d < 3; ++d) out[d] = s * ((-ca) * u(a, d) + u(bb, d));
         };
-        if (m == 0) { term(C / len(0), 0, cs(0), 1, F); return; }
-        if (m == N - 1) { term2(C / len(N - 2), cs(N - 3), N - 2, N - 3, F); return; }
+        if (m == 0) { term(Cst / len(0), 0, cs(0), 1, F); return; }
+        if (m == N - 1) { term2(Cst / len(N - 2), cs(N - 3), N - 2, N - 3, F); return; }
         if (N == 3) {
             term(1.0 / len(1), 1, cs(0), 0, A_);
             term2(1.0 / len(0), cs(0), 0, 1, B_);
-            for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d]);
+            for (int d = 0; d < 3; ++d) F[d] = Cst * (A_[d] + B_[d]);
             return;
         }
         if (m == 1) {
             term(1.0 / len(1), 1, cs(0), 0, A_);
             term2(1.0 / len(0), cs(0), 0, 1, B_);
             term(1.0 / len(1), 1, cs(1), 2, D_);
-            for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d] + D_[d]);
+            for (int d = 0; d < 3; ++d) F[d] = Cst * (A_[d] + B_[d] + D_[d]);
             return;
         }
         if (m == N - 2) {
             term(1.0 / len(N - 2), N - 2, cs(N - 3), N - 3, A_);
             term2(1.0 / len(N - 3), cs(N - 3), N - 3, N - 2, B_);
             term2(1.0 / len(N - 3), cs(N - 4), N - 3, N - 4, C_);
-            for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d] + C_[d]);
+            for (int d = 0; d < 3; ++d) F[d] = Cst * (A_[d] + B_[d] + C_[d]);
             return;
         }
         term(1.0 / len(m), m, cs(m - 1), m - 1, A_);
         term2(1.0 / len(m - 1), cs(m - 1), m - 1, m, B_);
         term2(1.0 / len(m - 1), cs(m - 2), m - 1, m - 2, C_);
         term(1.0 / len(m), m, cs(m), m + 1, D_);
-        for (int d = 0; d < 3; ++d) F[d] = C * (A_[d] + B_[d] + C_[d] + D_[d]);
+        for (int d = 0; d < 3; ++d) F[d] = Cst * (A_[d] + B_[d] + C_[d] + D_[d]);
     }
 
-    // Excluded-volume force (excluded_volume.cpp:60-89) on the own bead at sm.P:
+    // Excluded-volume force (excluded_volume.cpp:60-89) on the own bead at C:
     // F_mu = sum_nu -fac(r) (R_nu - R_mu) over |nu - mu| >= cd.
-    __device__ void ev_force(double F[3]) {
+    __device__ void ev_force(const double (*C)[BLOCK], double F[3]) {
         F[0] = F[1] = F[2] = 0.0;
         if (P.ev_type == 0 || P.ev_type == 4 || !own) return;
-        const double px = sm.P[0][tid], py = sm.P[1][tid], pz = sm.P[2][tid];
+        const double px = C[0][tid], py = C[1][tid], pz = C[2][tid];
         const double* phi = A.blob + P.phi_off;
         double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll 1
         for (int nu = 0; nu < N; ++nu) {
             const int sep = nu > tid ? nu - tid : tid - nu;
             if (sep == 0 || sep < P.cd) continue;
-            const double qx = sm.P[0][nu] - px, qy = sm.P[1][nu] - py, qz = sm.P[2][nu] - pz;
+            const double qx = C[0][nu] - px, qy = C[1][nu] - py, qz = C[2][nu] - pz;
             double sq = qx * qx + qy * qy + qz * qz;
             if (sq < 1.0e-12) sq = 1.0e-12;
             const double r = sqrt(sq);
@@ -320,21 +378,22 @@ struct Chain {
         F[0] = fx; F[1] = fy; F[2] = fz;
     }
 
-    // non-spring + external force at sm.P and time t (model.hpp:56-64)
-    __device__ void explicit_force(double t, double Fe[3]) {
+    // non-spring + external force at C and time t (model.hpp:56-64)
+    __device__ void explicit_force(const double (*C)[BLOCK], double t, double Fe[3]) {
         double Fev[3], Fb[3], Fx[3] = {0.0, 0.0, 0.0};
-        ev_force(Fev);
-        bending_force(Fb);
+        ev_force(C, Fev);
+        bending_force(C, Fb);
         if (P.n_ext > 0 && own) external_at(P, A.ext, A.blob, N, tid, t, Fx);
         for (int d = 0; d < 3; ++d) Fe[d] = (Fev[d] + Fb[d]) + Fx[d];
     }
 
-    // Dense RPY tensor into Dm (hydrodynamics.cpp:12-54, same entry formula and
-    // operand order), then its lower Cholesky factor in place (linalg.cpp:91-107,
-    // same per-entry sums, computed column by column). Returns false on a
-    // non-positive pivot, with its index/value in cnt.
-    __device__ bool dense_factor() {
+    // Dense RPY tensor into the workspace (hydrodynamics.cpp:12-54, same entry
+    // formula and operand order), then its lower Cholesky factor in place
+    // (linalg.cpp:91-107, same per-entry sums, computed column by column).
+    // Returns false on a non-positive pivot, with its index/value counted.
+    __device__ __noinline__ bool dense_factor() {
         const int n = n3;
+        double* Dm = ws + wl.dense;
         const double RPI = 1.7724538509055160273;
         const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
                      C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75, hstar = P.hstar;
@@ -383,8 +442,8 @@ struct Chain {
             }
             __syncthreads();
             if (!sm.ScalI[4]) {
-                cnt.fail_index = j;
-                cnt.fail_value = sm.Scal[15];
+                put(sm.cnt.fail_index, j);
+                put(sm.cnt.fail_value, sm.Scal[15]);
                 return false;
             }
             const double Ljj = Dm[(size_t)j * n + j];
@@ -398,36 +457,41 @@ struct Chain {
         return true;
     }
 
-    // y = L x with the factor in Dm (linalg.cpp:109-117), own rows.
-    __device__ void lower_matvec(const double x[3], double y[3]) {
-        constexpr int XB = XBufs<BLOCK>::value;
-        const int b = XB == 2 ? xpar : 0;
-        xpar ^= 1;
-        if constexpr (XB == 1) __syncthreads();
-        for (int d = 0; d < 3; ++d) sm.X[b][d][tid] = x[d];
+    // y = L x with the factor in the workspace (linalg.cpp:109-117), own rows.
+    __device__ __noinline__ void lower_matvec(const double x[3], double y[3]) {
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) sm.V[d][tid] = x[d];
         __syncthreads();
         y[0] = y[1] = y[2] = 0.0;
         if (!own) return;
         const int n = n3;
+        const double* Dm = ws + wl.dense;
         for (int p = 0; p < 3; ++p) {
             const int i = 3 * tid + p;
             const double* row = Dm + (size_t)i * n;
             double s = 0.0;
-            for (int j = 0; j <= i; ++j) s += row[j] * sm.X[b][j % 3][j / 3];
+            for (int j = 0; j <= i; ++j) s += row[j] * sm.V[j % 3][j / 3];
             y[p] = s;
         }
     }
 
-    // m-step Lanczos with full reorthogonalisation (hydrodynamics.cpp:146-220).
-    // Returns the Krylov dimension; Ritz values/vectors of the extremes out.
-    __device__ int lanczos(int m, bool warm, const double seed[3], const double u[3],
-                           double& rmin, double& rmax, double vmin[3], double vmax[3]) {
+    // default Lanczos seed sin(0.7654321 (i+1)) + 0.5 (hydrodynamics.cpp:83-87)
+    __device__ __forceinline__ void default_seed(double u[3]) {
+        for (int d = 0; d < 3; ++d) u[d] = own ? sin(0.7654321 * (3 * tid + d + 1)) + 0.5 : 0.0;
+    }
+
+    // m-step Lanczos with full reorthogonalisation (hydrodynamics.cpp:146-220)
+    // from the seed slot (warm) or the default seed. Returns the Krylov
+    // dimension; extremal Ritz values out, Ritz vectors into the seed slot as
+    // vmin + vmax (the next seed is built from their sum only).
+    __device__ int lanczos(int m, bool warm, double& rmin, double& rmax, double vsum[3]) {
         if (m > n3) m = n3;
+        double* Vb = ws + wl.V;
         double v[3], vprev[3] = {0.0, 0.0, 0.0}, w[3];
-        for (int d = 0; d < 3; ++d) v[d] = warm ? seed[d] : u[d];
+        if (warm) load(SLOT_SEED, v); else default_seed(v);
         double nv = sqrt(bsum(dot3(v, v)));
         if (!(nv > 0.0)) {
-            for (int d = 0; d < 3; ++d) v[d] = u[d];
+            default_seed(v);
             nv = sqrt(bsum(dot3(v, v)));
         }
         for (int d = 0; d < 3; ++d) v[d] /= nv;
@@ -438,7 +502,7 @@ struct Chain {
             if (own)
                 for (int d = 0; d < 3; ++d) Vb[(size_t)j * n + 3 * tid + d] = v[d];
             apply(v, w);
-            ++cnt.lanczos_iters;
+            bump(sm.cnt.lanczos_iters);
             const double a = bsum(dot3(v, w));
             if (tid == 0) sm.Alpha[j] = a;
             k = j + 1;
@@ -460,51 +524,39 @@ struct Chain {
             rmin = rmax = sm.Alpha[0];
             for (int d = 0; d < 3; ++d) {
                 const double v0 = own ? Vb[3 * tid + d] : 0.0;
-                vmin[d] = v0; vmax[d] = v0;
+                vsum[d] = v0 + v0;
             }
             return k;
         }
-        if (tid < 2) {
-            const double* al = sm.Alpha;
-            const double* be = sm.Beta;
-            double lo = al[0], hi_ = al[0];
-            for (int i = 0; i < k; ++i) {
-                const double rad = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < k - 1 ? fabs(be[i]) : 0.0);
-                if (al[i] - rad < lo) lo = al[i] - rad;
-                if (hi_ < al[i] + rad) hi_ = al[i] + rad;
-            }
-            const int target = tid == 0 ? 1 : k;
-            double a1 = lo, b1 = hi_;
-            for (int it = 0; it < 100; ++it) {
-                const double mid = 0.5 * (a1 + b1);
-                if (sturm_count(k, al, be, mid) >= target) b1 = mid; else a1 = mid;
-            }
-            const double r = 0.5 * (a1 + b1);
-            sm.Scal[tid] = r;
-            tridiag_eigenvector(k, al, be, r, sm.Tri[tid][0], sm.Tri[tid][1], sm.Tri[tid][2]);
-        }
+        double* tri = ws + wl.tri;
+        if (tid < 2) sm.Scal[tid] = ritz_extreme(k, sm.Alpha, sm.Beta, tid, tri);
         __syncthreads();
         rmin = sm.Scal[0];
         rmax = sm.Scal[1];
-        for (int d = 0; d < 3; ++d) { vmin[d] = 0.0; vmax[d] = 0.0; }
+        // lift the tridiagonal eigenvectors: v = V^T y (hydrodynamics.cpp:264-273)
+        double vmin[3] = {0.0, 0.0, 0.0}, vmax[3] = {0.0, 0.0, 0.0};
         if (own) {
+            const double* ymn = tri + 2 * LANCZOS_COLD;
+            const double* ymx = tri + 3 * LANCZOS_COLD + 2 * LANCZOS_COLD;
             for (int j = 0; j < k; ++j) {
-                const double ymn = sm.Tri[0][2][j], ymx = sm.Tri[1][2][j];
+                const double a0 = ymn[j], a1 = ymx[j];
                 for (int d = 0; d < 3; ++d) {
                     const double vj = Vb[(size_t)j * n + 3 * tid + d];
-                    vmin[d] += ymn * vj;
-                    vmax[d] += ymx * vj;
+                    vmin[d] += a0 * vj;
+                    vmax[d] += a1 * vj;
                 }
             }
         }
+        for (int d = 0; d < 3; ++d) vsum[d] = vmin[d] + vmax[d];
         return k;
     }
 
     // Chebyshev coefficients of sqrt on the interval (hydrodynamics.cpp:267-279)
-    // into sm.Coef[0..L]: one coefficient per thread.
+    // into the workspace: one coefficient per thread.
     __device__ void cheb_coefficients(int L, double da, double db) {
+        double* coef = ws + wl.coef;
+        double* aux = ws + wl.aux;
         __syncthreads();
-        double* aux = sm.aux();
         for (int k = tid; k <= L; k += BLOCK) aux[k] = sqrt(cos(PI * (k + 0.5) / (L + 1)) / da - db / da);
         __syncthreads();
         for (int j = tid; j <= L; j += BLOCK) {
@@ -512,9 +564,98 @@ struct Chain {
             for (int k = 0; k <= L; ++k) s += aux[k] * cos(j * (k + 0.5) * PI / (L + 1));
             s *= 2.0 / (L + 1);
             if (j == 0) s /= 2.0;
-            sm.Coef[j] = s;
+            coef[j] = s;
         }
         __syncthreads();
+    }
+
+    // B.x0 by the Chebyshev method with Lanczos bounds and FD control
+    // (hydrodynamics.cpp:224-337, mobility.cpp:64-80), x0 in SLOT_X0. Returns
+    // false when the fallback factorisation fails.
+    __device__ bool chebyshev_bx0(bool& seed_valid, double bx0[3]) {
+        StepLog& lg = sm.lg;
+        double rmin, rmax, vsum[3];
+        bool cold = !seed_valid;
+        int k = lanczos(seed_valid ? LANCZOS_WARM : LANCZOS_COLD, seed_valid, rmin, rmax, vsum);
+        if (seed_valid && k < 4) {
+            k = lanczos(LANCZOS_COLD, false, rmin, rmax, vsum);
+            cold = true;
+        }
+        if (cold) bump(sm.cnt.lanczos_cold); else bump(sm.cnt.lanczos_warm);
+        put(lg.lanczos_cold, cold ? 1 : 0);
+        put(lg.lanczos_steps, k);
+        {
+            // next seed: normalize(vmin + vmax + 0.01 u/|u|)  (hydrodynamics.cpp:294-311)
+            double u[3], s[3];
+            default_seed(u);
+            const double un = sqrt(bsum(dot3(u, u)));
+            for (int d = 0; d < 3; ++d) s[d] = vsum[d] + 0.01 * u[d] / un;
+            const double sn = sqrt(bsum(dot3(s, s)));
+            if (sn > 0.0 && isfinite(sn)) {
+                for (int d = 0; d < 3; ++d) s[d] = s[d] / sn;
+                save(SLOT_SEED, s);
+                seed_valid = true;
+            } else {
+                seed_valid = false;
+            }
+        }
+        double lmin = 0.5 * rmin;
+        double lmax = 1.02 * rmax;
+        if (lmin < 1.0e-12) lmin = 1.0e-12;
+        if (lmin >= lmax) lmin = lmax * 1.0e-6;
+        put(lg.lmin, lmin);
+        put(lg.lmax, lmax);
+
+        double x0[3], Dx0[3];
+        load(SLOT_X0, x0);
+        apply(x0, Dx0);
+        const double x0Dx0 = bsum(dot3(x0, Dx0));
+        int L = (int)(sqrt(lmax / lmin) * P.mult + 0.5) + 1;
+        if (L > MAXCHEB) L = MAXCHEB;
+        if (L < 2) L = 2;
+        const double da = 2.0 / (lmax - lmin);
+        const double db = -(lmax + lmin) / (lmax - lmin);
+        const double* coef = ws + wl.coef;
+        double fd = 0.0;
+        for (;;) {
+            cheb_coefficients(L, da, db);
+            double tprev[3], tcur[3], res[3];
+            for (int d = 0; d < 3; ++d) {
+                res[d] = coef[0] * x0[d];
+                tprev[d] = x0[d];
+                tcur[d] = da * Dx0[d] + db * x0[d];
+                res[d] += coef[1] * tcur[d];
+            }
+            for (int l = 2; l <= L; ++l) {
+                double Dv[3];
+                apply(tcur, Dv);
+                const double cl = coef[l];
+                for (int d = 0; d < 3; ++d) {
+                    double tn = da * Dv[d] + db * tcur[d];
+                    tn = 2.0 * tn - tprev[d];
+                    tprev[d] = tcur[d];
+                    tcur[d] = tn;
+                    res[d] += cl * tcur[d];
+                }
+            }
+            bump(sm.cnt.cheb_attempts);
+            bump(sm.cnt.cheb_terms, L);
+            bump32(lg.cheb_attempts);
+            fd = fabs(bsum(dot3(res, res)) - x0Dx0) / x0Dx0;
+            for (int d = 0; d < 3; ++d) bx0[d] = res[d];
+            if (fd <= P.fd_tol || L >= MAXCHEB) break;
+            const int grow = (L / 4 > 1) ? L / 4 : 1;
+            L = (L + grow < MAXCHEB) ? L + grow : MAXCHEB;
+        }
+        put(lg.cheb_L, L);
+        put(lg.fd_err, fd);
+        if (!(fd <= P.fd_tol)) {
+            bump(sm.cnt.fallbacks);
+            put(lg.fallback, 1);
+            if (!dense_factor()) return false;
+            lower_matvec(x0, bx0);
+        }
+        return true;
     }
 
     __device__ void run(int ti) {
@@ -523,15 +664,6 @@ struct Chain {
             for (int d = 0; d < 3; ++d) R[d] = A.R[P.R_off + 3 * tid + d];
         int32_t ix = A.rng[ti].ix, iy = A.rng[ti].iy, rseed = A.rng[ti].seed;
         bool rng_ready = false;
-
-        const bool cheb = hi && P.hi_method == 0;
-        double u[3] = {0.0, 0.0, 0.0}, un = 0.0;
-        if (cheb) {
-            if (own)
-                for (int d = 0; d < 3; ++d) u[d] = sin(0.7654321 * (3 * tid + d + 1)) + 0.5;
-            un = sqrt(bsum(dot3(u, u)));
-        }
-        double seedv[3] = {0.0, 0.0, 0.0};
         bool seed_valid = false;
         int status = ST_OK;
 
@@ -540,15 +672,14 @@ struct Chain {
         const double dt = A.sim_override ? A.dt : P.dt;
         const double tol = A.sim_override ? A.tol : P.tol;
         const int com = A.sim_override ? A.com : P.com;
-        const double sqrt_dt = sqrt(dt);
         const double s2i = 1.0 / sqrt(2.0);
         StepLog* logp = (A.log && P.log_off >= 0) ? A.log + P.log_off : nullptr;
+        StepLog& lg = sm.lg;
         int step = 0;
 
         for (double t = t0; t <= t1 + dt / 2.0; t += dt) {
-            StepLog lg{};
-            lg.t = t;
-            // recenter (integrator.cpp:88-93): sequential sums as the reference
+            if (tid == 0) { lg = StepLog{}; lg.t = t; }
+            // recenter (integrator.cpp:88-93): sequential sums, as the reference
             if (com) {
                 __syncthreads();
                 for (int d = 0; d < 3; ++d) sm.P[d][tid] = R[d];
@@ -562,229 +693,195 @@ struct Chain {
                 if (own)
                     for (int d = 0; d < 3; ++d) R[d] = R[d] - sm.Scal[8 + d];
             }
-            __syncthreads();
-            for (int d = 0; d < 3; ++d) { sm.R[d][tid] = R[d]; sm.P[d][tid] = R[d]; }
-            __syncthreads();
-            stage_mixed();
+            stage_R(R);
 
-            // forces at R (integrator.cpp:70-76)
-            double bond[3] = {0.0, 0.0, 0.0};
-            if (tid < N - 1)
-                for (int d = 0; d < 3; ++d) bond[d] = sm.P[d][tid + 1] - R[d];
-            double Fc[3], Fs[3], Fe[3];
-            spring_bead_force(bond, Fc, Fs);
-            explicit_force(t, Fe);
+            // forces at R (integrator.cpp:70-76), parked in the state slots
+            {
+                double bond[3] = {0.0, 0.0, 0.0}, Fc[3], F[3];
+                if (tid < N - 1)
+                    for (int d = 0; d < 3; ++d) bond[d] = sm.R[d][tid + 1] - R[d];
+                spring_bead_force(bond, Fc, F);
+                save(SLOT_FS, F);
+                explicit_force(sm.R, t, F);
+                save(SLOT_FE, F);
+            }
 
-            // mobility at R; Cholesky method factors here (mobility.cpp:39-51)
+            // mobility at R; the Cholesky method factors here (mobility.cpp:39-51)
             if (hi && P.hi_method == 1) {
                 if (!dense_factor()) { status = ST_NOT_PD; break; }
             }
 
             // B.dW (mobility.cpp:58-88)
             if (!rng_ready) { rng_maybe_init(ix, iy, rseed); rng_ready = true; }
-            double x0[3] = {0.0, 0.0, 0.0};
+            double bx0[3] = {0.0, 0.0, 0.0};
             {
+                double x0[3] = {0.0, 0.0, 0.0};
                 int32_t nix = ix, niy = iy;
                 if (own) bead_noise(A.jx, A.jy, tid, ix, iy, x0, nix, niy);
                 if (tid == N - 1) { sm.ScalI[0] = nix; sm.ScalI[1] = niy; }
                 __syncthreads();
                 ix = sm.ScalI[0];
                 iy = sm.ScalI[1];
-            }
-            double bx0[3] = {0.0, 0.0, 0.0};
-            if (!hi) {
-                for (int d = 0; d < 3; ++d) bx0[d] = x0[d];
-            } else if (P.hi_method == 1) {
-                lower_matvec(x0, bx0);
-            } else if (P.hi_method == 0) {
-                // spectral bounds (hydrodynamics.cpp:224-264)
-                double vmin[3], vmax[3], rmin, rmax;
-                bool cold = !seed_valid;
-                int k = lanczos(seed_valid ? LANCZOS_WARM : LANCZOS_COLD, seed_valid, seedv, u,
-                                rmin, rmax, vmin, vmax);
-                if (seed_valid && k < 4) {
-                    k = lanczos(LANCZOS_COLD, false, seedv, u, rmin, rmax, vmin, vmax);
-                    cold = true;
-                }
-                if (cold) ++cnt.lanczos_cold; else ++cnt.lanczos_warm;
-                lg.lanczos_cold = cold ? 1 : 0;
-                lg.lanczos_steps = k;
-                {
-                    double s[3];
-                    for (int d = 0; d < 3; ++d) s[d] = vmin[d] + vmax[d] + 0.01 * u[d] / un;
-                    const double sn = sqrt(bsum(dot3(s, s)));
-                    if (sn > 0.0 && isfinite(sn)) {
-                        for (int d = 0; d < 3; ++d) seedv[d] = s[d] / sn;
-                        seed_valid = true;
-                    } else {
-                        seed_valid = false;
-                    }
-                }
-                double lmin = 0.5 * rmin;
-                double lmax = 1.02 * rmax;
-                if (lmin < 1.0e-12) lmin = 1.0e-12;
-                if (lmin >= lmax) lmin = lmax * 1.0e-6;
-                lg.lmin = lmin;
-                lg.lmax = lmax;
-
-                // Chebyshev series with FD control (hydrodynamics.cpp:284-337)
-                double Dx0[3];
-                apply(x0, Dx0);
-                const double x0Dx0 = bsum(dot3(x0, Dx0));
-                int L = (int)(sqrt(lmax / lmin) * P.mult + 0.5) + 1;
-                if (L > MAXCHEB) L = MAXCHEB;
-                if (L < 2) L = 2;
-                const double da = 2.0 / (lmax - lmin);
-                const double db = -(lmax + lmin) / (lmax - lmin);
-                double fd = 0.0;
-                for (;;) {
-                    cheb_coefficients(L, da, db);
-                    double tprev[3], tcur[3], res[3];
-                    for (int d = 0; d < 3; ++d) {
-                        res[d] = sm.Coef[0] * x0[d];
-                        tprev[d] = x0[d];
-                        tcur[d] = da * Dx0[d] + db * x0[d];
-                        res[d] += sm.Coef[1] * tcur[d];
-                    }
-                    for (int l = 2; l <= L; ++l) {
-                        double Dv[3];
-                        apply(tcur, Dv);
-                        for (int d = 0; d < 3; ++d) {
-                            double tn = da * Dv[d] + db * tcur[d];
-                            tn = 2.0 * tn - tprev[d];
-                            tprev[d] = tcur[d];
-                            tcur[d] = tn;
-                            res[d] += sm.Coef[l] * tcur[d];
-                        }
-                    }
-                    ++cnt.cheb_attempts;
-                    cnt.cheb_terms += L;
-                    ++lg.cheb_attempts;
-                    fd = fabs(bsum(dot3(res, res)) - x0Dx0) / x0Dx0;
-                    for (int d = 0; d < 3; ++d) bx0[d] = res[d];
-                    if (fd <= P.fd_tol || L >= MAXCHEB) break;
-                    const int grow = (L / 4 > 1) ? L / 4 : 1;
-                    L = (L + grow < MAXCHEB) ? L + grow : MAXCHEB;
-                }
-                lg.cheb_L = L;
-                lg.fd_err = fd;
-                if (!(fd <= P.fd_tol)) {
-                    ++cnt.fallbacks;
-                    lg.fallback = 1;
-                    if (!dense_factor()) { status = ST_NOT_PD; break; }
+                if (!hi) {
+                    for (int d = 0; d < 3; ++d) bx0[d] = x0[d];
+                } else if (P.hi_method == 1) {
                     lower_matvec(x0, bx0);
+                } else if (P.hi_method == 0) {
+                    save(SLOT_X0, x0);
+                } else {
+                    status = ST_EXACT_SQRT;
+                    break;
                 }
-            } else {
-                status = ST_EXACT_SQRT;
-                break;
             }
-            double dW[3];
-            for (int d = 0; d < 3; ++d) dW[d] = sqrt_dt * bx0[d];
-
-            double K0[9], K1[9];
-            kappa_at(P, A.blob, t, K0);
-            kappa_at(P, A.blob, t + dt, K1);
+            if (hi && P.hi_method == 0) {
+                if (!chebyshev_bx0(seed_valid, bx0)) { status = ST_NOT_PD; break; }
+            }
+            {
+                const double sqrt_dt = sqrt(dt);
+                double dW[3];
+                for (int d = 0; d < 3; ++d) dW[d] = sqrt_dt * bx0[d];
+                save(SLOT_DW, dW);
+            }
 
             // predictor (integrator.cpp:96-101)
-            double v[3], DF[3], fl0[3], Rp[3];
-            for (int d = 0; d < 3; ++d) v[d] = Fs[d] + Fe[d];
-            apply(v, DF);
-            for (int d = 0; d < 3; ++d) fl0[d] = K0[3 * d] * R[0] + K0[3 * d + 1] * R[1] + K0[3 * d + 2] * R[2];
-            for (int d = 0; d < 3; ++d) Rp[d] = own ? R[d] + dt * (fl0[d] + 0.25 * DF[d]) + s2i * dW[d] : 0.0;
+            double Rp[3];
+            {
+                double Fs[3], Fe[3], v[3], DF[3], dW[3];
+                load(SLOT_FS, Fs);
+                load(SLOT_FE, Fe);
+                for (int d = 0; d < 3; ++d) v[d] = Fs[d] + Fe[d];
+                apply(v, DF);
+                load(SLOT_DW, dW);
+                for (int d = 0; d < 3; ++d) {
+                    const int r = 3 * d;
+                    double K[3];
+                    kappa_row(t, r, K);
+                    const double fl0 = K[0] * R[0] + K[1] * R[1] + K[2] * R[2];
+                    Rp[d] = own ? R[d] + dt * (fl0 + 0.25 * DF[d]) + s2i * dW[d] : 0.0;
+                }
+            }
 
             // upsilon (integrator.cpp:107-119)
-            __syncthreads();
-            for (int d = 0; d < 3; ++d) sm.P[d][tid] = Rp[d];
-            __syncthreads();
-            double Fep[3], flow[3], DFs[3], ups[3];
-            explicit_force(t + dt, Fep);
-            for (int d = 0; d < 3; ++d) {
-                const double f1 = K1[3 * d] * Rp[0] + K1[3 * d + 1] * Rp[1] + K1[3 * d + 2] * Rp[2];
-                flow[d] = 0.5 * (fl0[d] + f1);
+            double ups[3];
+            {
+                __syncthreads();
+                for (int d = 0; d < 3; ++d) sm.P[d][tid] = Rp[d];
+                __syncthreads();
+                double Fep[3], Fe[3], Fs[3], DFs[3], DFe[3], v[3], dW[3];
+                explicit_force(sm.P, t + dt, Fep);
+                load(SLOT_FE, Fe);
+                for (int d = 0; d < 3; ++d) v[d] = Fe[d] + Fep[d];
+                load(SLOT_FS, Fs);
+                apply(Fs, DFs);
+                apply(v, DFe);
+                load(SLOT_DW, dW);
+                for (int d = 0; d < 3; ++d) {
+                    double K0[3], K1[3];
+                    kappa_row(t, 3 * d, K0);
+                    kappa_row(t + dt, 3 * d, K1);
+                    const double f0 = K0[0] * R[0] + K0[1] * R[1] + K0[2] * R[2];
+                    const double f1 = K1[0] * Rp[0] + K1[1] * Rp[1] + K1[2] * Rp[2];
+                    const double flow = 0.5 * (f0 + f1);
+                    ups[d] = own ? R[d] + dt * (flow + 0.125 * DFs[d] + 0.125 * DFe[d]) + s2i * dW[d] : 0.0;
+                }
             }
-            apply(Fs, DFs);
-            for (int d = 0; d < 3; ++d) v[d] = Fe[d] + Fep[d];
-            apply(v, DF);
-            for (int d = 0; d < 3; ++d)
-                ups[d] = own ? R[d] + dt * (flow[d] + 0.125 * DFs[d] + 0.125 * DF[d]) + s2i * dW[d] : 0.0;
 
             // implicit connector solve (integrator.cpp:125-165)
-            __syncthreads();
-            for (int d = 0; d < 3; ++d) sm.V[d][tid] = ups[d];
-            __syncthreads();
-            double dups[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
-            if (tid < N - 1)
-                for (int d = 0; d < 3; ++d) {
-                    dups[d] = sm.V[d][tid + 1] - ups[d];
-                    cb[d] = sm.P[d][tid + 1] - Rp[d];
-                }
-            const double anchor0 = sm.P[0][0], anchor1 = sm.P[1][0], anchor2 = sm.P[2][0];
-            double Rprev[3] = {Rp[0], Rp[1], Rp[2]}, Rc[3] = {0.0, 0.0, 0.0};
-            const int max_iter = 100 * N;
-            int iter = 0;
-            bool fail = false;
-            for (; iter <= max_iter; ++iter) {
-                double Fb[3], DFc[3];
-                spring_bead_force(cb, Fc, Fb);
-                apply(Fb, DFc);
-                ++cnt.corrector_iters;
-                ++lg.corrector_iters;
+            {
+                double dups[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
+                if (tid < N - 1)
+                    for (int d = 0; d < 3; ++d) cb[d] = sm.P[d][tid + 1] - Rp[d];
+                const double anchor0 = sm.P[0][0], anchor1 = sm.P[1][0], anchor2 = sm.P[2][0];
                 __syncthreads();
-                for (int d = 0; d < 3; ++d) sm.V[d][tid] = DFc[d];
+                for (int d = 0; d < 3; ++d) sm.V[d][tid] = ups[d];
                 __syncthreads();
-                int sst = ST_OK;
-                if (tid < N - 1) {
-                    double g[3];
-                    for (int d = 0; d < 3; ++d)
-                        g[d] = dups[d] + (0.125 * dt) * (sm.V[d][tid + 1] - DFc[d]) + (0.25 * dt) * Fc[d];
-                    sst = solve_connector(g, dt, P.spring_type, P.sqrtb, P.q0, cb);
-                }
-                for (int d = 0; d < 3; ++d) sm.B[d][tid] = cb[d];
-                if (sst != ST_OK) sm.ScalI[2] = sst;
-                const int any_fail = __syncthreads_or(sst != ST_OK);
-                if (any_fail) { status = sm.ScalI[2]; fail = true; break; }
-                if (tid < 3) {
-                    double acc = tid == 0 ? anchor0 : (tid == 1 ? anchor1 : anchor2);
-                    sm.V[tid][0] = acc;
-                    for (int mu = 0; mu < N - 1; ++mu) {
-                        acc = acc + sm.B[tid][mu];
-                        sm.V[tid][mu + 1] = acc;
+                if (tid < N - 1)
+                    for (int d = 0; d < 3; ++d) dups[d] = sm.V[d][tid + 1] - ups[d];
+                double Rprev[3] = {Rp[0], Rp[1], Rp[2]};
+                const int max_iter = 100 * N;
+                int iter = 0;
+                bool fail = false;
+                for (; iter <= max_iter; ++iter) {
+                    double Fc[3], Fb[3], DFc[3];
+                    spring_bead_force(cb, Fc, Fb);
+                    apply(Fb, DFc);
+                    bump(sm.cnt.corrector_iters);
+                    bump32(lg.corrector_iters);
+                    __syncthreads();
+                    for (int d = 0; d < 3; ++d) sm.V[d][tid] = DFc[d];
+                    __syncthreads();
+                    int sst = ST_OK;
+                    if (tid < N - 1) {
+                        double g[3];
+                        for (int d = 0; d < 3; ++d)
+                            g[d] = dups[d] + (0.125 * dt) * (sm.V[d][tid + 1] - DFc[d]) + (0.25 * dt) * Fc[d];
+                        sst = solve_connector_ool(g[0], g[1], g[2], dt, P.spring_type, P.sqrtb, P.q0, cb);
                     }
+                    for (int d = 0; d < 3; ++d) sm.B[d][tid] = cb[d];
+                    if (sst != ST_OK) sm.ScalI[2] = sst;
+                    if (__syncthreads_or(sst != ST_OK)) { status = sm.ScalI[2]; fail = true; break; }
+                    // positions from the anchor by the reference's sequential sum
+                    if (tid < 3) {
+                        double acc = tid == 0 ? anchor0 : (tid == 1 ? anchor1 : anchor2);
+                        sm.P[tid][0] = acc;
+                        for (int mu = 0; mu < N - 1; ++mu) {
+                            acc = acc + sm.B[tid][mu];
+                            sm.P[tid][mu + 1] = acc;
+                        }
+                    }
+                    __syncthreads();
+                    double dv[3] = {0.0, 0.0, 0.0};
+                    if (own)
+                        for (int d = 0; d < 3; ++d) { R[d] = sm.P[d][tid]; dv[d] = R[d] - Rprev[d]; }
+                    const double increment = sqrt(bsum(dot3(dv, dv))) / N;
+                    if (increment < tol) break;
+                    for (int d = 0; d < 3; ++d) Rprev[d] = R[d];
                 }
-                __syncthreads();
-                double dv[3] = {0.0, 0.0, 0.0};
-                if (own)
-                    for (int d = 0; d < 3; ++d) { Rc[d] = sm.V[d][tid]; dv[d] = Rc[d] - Rprev[d]; }
-                const double increment = sqrt(bsum(dot3(dv, dv))) / N;
-                if (increment < tol) break;
-                for (int d = 0; d < 3; ++d) Rprev[d] = Rc[d];
+                if (fail) break;
+                if (iter > max_iter) bump(sm.cnt.nonconverged);
             }
-            if (fail) break;
-            if (iter > max_iter) ++cnt.nonconverged;
-            for (int d = 0; d < 3; ++d) R[d] = Rc[d];
-            ++cnt.steps;
+            bump(sm.cnt.steps);
             if (logp && step < P.log_cap && tid == 0) logp[step] = lg;
             ++step;
         }
 
         if (own)
             for (int d = 0; d < 3; ++d) A.R[P.R_off + 3 * tid + d] = R[d];
+        __syncthreads();
         if (tid == 0) {
             A.rng[ti].ix = ix;
             A.rng[ti].iy = iy;
             A.rng[ti].seed = rseed;
             A.status[ti] = status;
-            A.cnt[ti] = cnt;
+            A.cnt[ti] = sm.cnt;
         }
     }
+
+    // row r..r+2 of kappa(t) (flow.cpp:5-26)
+    __device__ __forceinline__ void kappa_row(double t, int r, double K[3]) {
+        if (P.flow_const || P.flow_n == 0) {
+            K[0] = P.kappa0[r]; K[1] = P.kappa0[r + 1]; K[2] = P.kappa0[r + 2];
+            return;
+        }
+        double Kf[9];
+        kappa_at(P, A.blob, t, Kf);
+        K[0] = Kf[r]; K[1] = Kf[r + 1]; K[2] = Kf[r + 2];
+    }
 };
+
+template <int BLOCK, bool MIXED>
+__device__ __forceinline__ void init_counters(Smem<BLOCK, MIXED>& sm) {
+    if (threadIdx.x == 0) { sm.cnt = Counters{}; sm.lg = StepLog{}; }
+    __syncthreads();
+}
 
 template <int BLOCK, bool MIXED>
 __global__ void __launch_bounds__(BLOCK) k_integrate(KArgs a) {
     __shared__ Smem<BLOCK, MIXED> sm;
     const int ti = a.list[blockIdx.x];
-    const TrajParams P = a.tp[ti];
-    Chain<BLOCK, MIXED> c(sm, P, a);
+    const TrajParams* __restrict__ P = a.tp + ti;
+    init_counters(sm);
+    Chain<BLOCK, MIXED> c(sm, *P, a);
     c.run(ti);
 }
 
@@ -792,26 +889,27 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(KArgs a) {
 enum : int { FORCE_TOTAL = 0, FORCE_SPRING = 1, FORCE_BENDING = 2, FORCE_EV = 3 };
 
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_forces(KArgs a, int which) {
+__global__ void __launch_bounds__(BLOCK) k_forces(KArgs a, int which, double* out) {
     __shared__ Smem<BLOCK, false> sm;
     const int ti = blockIdx.x;
-    const TrajParams P = a.tp[0];
-    Chain<BLOCK, false> c(sm, P, a);
-    const int N = P.N, tid = threadIdx.x;
+    const TrajParams* __restrict__ P = a.tp;
+    init_counters(sm);
+    Chain<BLOCK, false> c(sm, *P, a);
+    const int N = P->N, tid = threadIdx.x;
     const bool own = tid < N;
     const int64_t off = (int64_t)ti * 3 * N;
     double R[3] = {0.0, 0.0, 0.0};
     if (own)
         for (int d = 0; d < 3; ++d) R[d] = a.R[off + 3 * tid + d];
-    for (int d = 0; d < 3; ++d) { sm.R[d][tid] = R[d]; sm.P[d][tid] = R[d]; }
+    for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
     __syncthreads();
     double bond[3] = {0.0, 0.0, 0.0};
     if (tid < N - 1)
-        for (int d = 0; d < 3; ++d) bond[d] = sm.P[d][tid + 1] - R[d];
+        for (int d = 0; d < 3; ++d) bond[d] = sm.R[d][tid + 1] - R[d];
     double Fc[3], Fs[3], Fev[3], Fb[3], F[3];
     c.spring_bead_force(bond, Fc, Fs);
-    c.ev_force(Fev);
-    c.bending_force(Fb);
+    c.ev_force(sm.R, Fev);
+    c.bending_force(sm.R, Fb);
     for (int d = 0; d < 3; ++d) {
         if (which == FORCE_SPRING) F[d] = Fs[d];
         else if (which == FORCE_BENDING) F[d] = Fb[d];
@@ -819,11 +917,11 @@ __global__ void __launch_bounds__(BLOCK) k_forces(KArgs a, int which) {
         else F[d] = Fs[d] + (Fev[d] + Fb[d]);   // bindings/module.cpp:67-76
     }
     if (own)
-        for (int d = 0; d < 3; ++d) a.ws[off + 3 * tid + d] = F[d];
+        for (int d = 0; d < 3; ++d) out[off + 3 * tid + d] = F[d];
 }
 
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_noise(KArgs a, int N, int n_steps) {
+__global__ void __launch_bounds__(BLOCK) k_noise(KArgs a, int N, int n_steps, double* out) {
     __shared__ int32_t s[2];
     const int ti = blockIdx.x, tid = threadIdx.x;
     int32_t ix = a.rng[ti].ix, iy = a.rng[ti].iy, seed = a.rng[ti].seed;
@@ -834,7 +932,7 @@ __global__ void __launch_bounds__(BLOCK) k_noise(KArgs a, int N, int n_steps) {
         if (tid < N) {
             bead_noise(a.jx, a.jy, tid, ix, iy, X, nix, niy);
             for (int d = 0; d < 3; ++d)
-                a.ws[((int64_t)ti * n_steps + st) * 3 * N + 3 * tid + d] = X[d];
+                out[((int64_t)ti * n_steps + st) * 3 * N + 3 * tid + d] = X[d];
         }
         if (tid == N - 1) { s[0] = nix; s[1] = niy; }
         __syncthreads();
@@ -846,26 +944,26 @@ __global__ void __launch_bounds__(BLOCK) k_noise(KArgs a, int N, int n_steps) {
 }
 
 template <int BLOCK, bool MIXED>
-__global__ void __launch_bounds__(BLOCK) k_rpy_apply(KArgs a) {
+__global__ void __launch_bounds__(BLOCK) k_rpy_apply(KArgs a, const double* x_in, double* out) {
     __shared__ Smem<BLOCK, MIXED> sm;
     const int ti = blockIdx.x;
-    const TrajParams P = a.tp[0];
-    Chain<BLOCK, MIXED> c(sm, P, a);
-    const int N = P.N, tid = threadIdx.x;
+    const TrajParams* __restrict__ P = a.tp;
+    init_counters(sm);
+    Chain<BLOCK, MIXED> c(sm, *P, a);
+    const int N = P->N, tid = threadIdx.x;
     const int64_t off = (int64_t)ti * 3 * N;
-    double x[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < 3; ++d) sm.R[d][tid] = tid < N ? a.R[off + 3 * tid + d] : 0.0;
+    double x[3] = {0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0};
     if (tid < N)
-        for (int d = 0; d < 3; ++d) x[d] = a.blob[off + 3 * tid + d];
-    __syncthreads();
-    c.stage_mixed();
+        for (int d = 0; d < 3; ++d) { R[d] = a.R[off + 3 * tid + d]; x[d] = x_in[off + 3 * tid + d]; }
+    c.stage_R(R);
     double y[3];
     c.apply(x, y);
     if (tid < N)
-        for (int d = 0; d < 3; ++d) a.ws[off + 3 * tid + d] = y[d];
+        for (int d = 0; d < 3; ++d) out[off + 3 * tid + d] = y[d];
 }
 
 }  // namespace bdgpu
+
 
 // =========================================================================
 // host side
@@ -1104,7 +1202,7 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
         tps[i].R_off = roff;
         tps[i].ws_off = wsoff;
         roff += 3 * N;
-        if (phys[i].hstar > 0.0) wsoff += ws_lanczos_doubles(N) + ws_dense_doubles(N);
+        wsoff += ws_layout(N, phys[i].hstar > 0.0).total;
         if (log && log_stride > 0) {
             tps[i].log_off = (int64_t)i * log_stride;
             tps[i].log_cap = log_stride;
@@ -1193,7 +1291,7 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
     TrajParams proto;
     if (int rc = pack_traj(*phys, s0, N, blob, ext, proto)) return rc;
     std::vector<TrajParams> tps(n_traj, proto);
-    const int64_t wsper = phys->hstar > 0.0 ? ws_lanczos_doubles(N) + ws_dense_doubles(N) : 0;
+    const int64_t wsper = ws_layout(N, phys->hstar > 0.0).total;
     std::vector<int32_t> list(n_traj);
     for (int i = 0; i < n_traj; ++i) {
         tps[i].R_off = (int64_t)i * 3 * N;
@@ -1339,11 +1437,11 @@ static int forces_common(int n_traj, int nbeads, const double* R, const bdsim_ph
     a.ext = as<ExtEntry>(g.ext);
     dim3 grid(n_traj);
     switch (block_for(nbeads)) {
-        case 64: hipLaunchKernelGGL(k_forces<64>, grid, dim3(64), 0, g.stream, a, which); break;
-        case 128: hipLaunchKernelGGL(k_forces<128>, grid, dim3(128), 0, g.stream, a, which); break;
-        case 256: hipLaunchKernelGGL(k_forces<256>, grid, dim3(256), 0, g.stream, a, which); break;
-        case 512: hipLaunchKernelGGL(k_forces<512>, grid, dim3(512), 0, g.stream, a, which); break;
-        default: hipLaunchKernelGGL(k_forces<1024>, grid, dim3(1024), 0, g.stream, a, which); break;
+        case 64: hipLaunchKernelGGL(k_forces<64>, grid, dim3(64), 0, g.stream, a, which, a.ws); break;
+        case 128: hipLaunchKernelGGL(k_forces<128>, grid, dim3(128), 0, g.stream, a, which, a.ws); break;
+        case 256: hipLaunchKernelGGL(k_forces<256>, grid, dim3(256), 0, g.stream, a, which, a.ws); break;
+        case 512: hipLaunchKernelGGL(k_forces<512>, grid, dim3(512), 0, g.stream, a, which, a.ws); break;
+        default: hipLaunchKernelGGL(k_forces<1024>, grid, dim3(1024), 0, g.stream, a, which, a.ws); break;
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(F, g.ws.p, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
@@ -1425,11 +1523,11 @@ int bdsim_raw_noise_batch(int n_traj, int nbeads, int n_steps, bdsim_rng_state* 
     a.jy = g.jy;
     dim3 grid(n_traj);
     switch (block_for(nbeads)) {
-        case 64: hipLaunchKernelGGL(k_noise<64>, grid, dim3(64), 0, g.stream, a, nbeads, n_steps); break;
-        case 128: hipLaunchKernelGGL(k_noise<128>, grid, dim3(128), 0, g.stream, a, nbeads, n_steps); break;
-        case 256: hipLaunchKernelGGL(k_noise<256>, grid, dim3(256), 0, g.stream, a, nbeads, n_steps); break;
-        case 512: hipLaunchKernelGGL(k_noise<512>, grid, dim3(512), 0, g.stream, a, nbeads, n_steps); break;
-        default: hipLaunchKernelGGL(k_noise<1024>, grid, dim3(1024), 0, g.stream, a, nbeads, n_steps); break;
+        case 64: hipLaunchKernelGGL(k_noise<64>, grid, dim3(64), 0, g.stream, a, nbeads, n_steps, a.ws); break;
+        case 128: hipLaunchKernelGGL(k_noise<128>, grid, dim3(128), 0, g.stream, a, nbeads, n_steps, a.ws); break;
+        case 256: hipLaunchKernelGGL(k_noise<256>, grid, dim3(256), 0, g.stream, a, nbeads, n_steps, a.ws); break;
+        case 512: hipLaunchKernelGGL(k_noise<512>, grid, dim3(512), 0, g.stream, a, nbeads, n_steps, a.ws); break;
+        default: hipLaunchKernelGGL(k_noise<1024>, grid, dim3(1024), 0, g.stream, a, nbeads, n_steps, a.ws); break;
     }
     HIP_TRY(hipGetLastError());
     if (nd) HIP_TRY(hipMemcpyAsync(X, g.ws.p, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
@@ -1466,8 +1564,8 @@ int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision, c
     dim3 grid(n_traj);
     const bool mixed = precision == BDSIM_PRECISION_MIXED;
 #define LAUNCH_A(BS)                                                                         \
-    if (mixed) hipLaunchKernelGGL((k_rpy_apply<BS, true>), grid, dim3(BS), 0, g.stream, a);  \
-    else hipLaunchKernelGGL((k_rpy_apply<BS, false>), grid, dim3(BS), 0, g.stream, a);
+    if (mixed) hipLaunchKernelGGL((k_rpy_apply<BS, true>), grid, dim3(BS), 0, g.stream, a, a.blob, a.ws);  \
+    else hipLaunchKernelGGL((k_rpy_apply<BS, false>), grid, dim3(BS), 0, g.stream, a, a.blob, a.ws);
     switch (block_for(nbeads)) {
         case 64: LAUNCH_A(64); break;
         case 128: LAUNCH_A(128); break;

@@ -1,0 +1,276 @@
+"""Benchmark of the bdsim ChainIntegrator step on one MI355X.
+
+Workload (the headline metric's configuration): 4096 chains of N=50 beads,
+FENE springs b=100 (sqrtb=10), RPY hydrodynamics h*=0.3 with the Chebyshev
+square root (multiplier 1, FD tolerance 0.0025), SDK excluded volume (d*=1,
+cutoffs 0.7/1.5, phi=0), simple shear with Wi in {0.1, 1, 10, 100} assigned
+round robin, dt=1e-3, implicit tolerance 1e-3, centre-of-mass update on.
+The shear rate is Wi / tau_1 with tau_1 = 1/sin^2(pi/2N), the longest Rouse
+relaxation time in code units -- the reference's own way of placing a run at a
+Weissenberg number (python/bdsim/dynamics.py:583-606,
+validation/hi_nan_probe.py:103).
+Chain i uses Rng(seed=i) and a Gaussian random-walk start (bond std 1) drawn
+from numpy's generator with seed i. Data is synthetic.
+
+A "step" is one integrator step of every chain. W warm-up steps run as one
+integrate segment, then K timed steps run as ONE further segment (one cold
+Lanczos start, as a single integrate call does), with positions, RNG states and
+workspace resident in HBM. `value` is the mixed-precision form's
+bead-steps/s = 4096 * 50 * K / seconds; the plain FP64 form is timed the same
+way and reported beside it.
+
+roofline: algorithmic FLOPs of one launch = sum over chains of 18 N^2 per
+mobility product (a dense 3N x 3N mat-vec) times the mat-vecs the run itself
+counted; achieved = that / kernel time measured with HIP events on the batch's
+stream. Peak: MI355X vector FP32 157.3 TFLOP/s (mixed) / FP64 78.6 TFLOP/s.
+
+cpu_baseline: the reference core (oracle/_ref, built from the reference
+sources) -- or, where that build is absent, the FP64 C restatement -- integrating
+a bounded sample of the same workload in one process per host core.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from ctypes import byref
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from bdsim_amd import _abi as A  # noqa: E402
+
+N_BEADS = 50
+N_TRAJ = 4096
+WI = [0.1, 1.0, 10.0, 100.0]
+PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}
+
+
+def headline_phys():
+    c = A.CPhys()
+    c.spring_type = 2
+    c.sqrtb = 10.0
+    c.natural_length = 0.0
+    c.ev_type = 3
+    c.zstar = 0.0
+    c.dstar = 1.0
+    c.min_cutoff = 0.7
+    c.max_cutoff = 1.5
+    c.contour_dist_for_ev = 1
+    c.bend_type = 0
+    c.flow.constant = 1
+    c.number_of_beads = N_BEADS
+    c.hstar = 0.3
+    c.hi_method = 0
+    c.ncheb_multiplier = 1.0
+    c.fd_err_max = 0.0025
+    c.equilibration = 0
+    return c
+
+
+def start_configs(n_traj, N):
+    R = np.empty((n_traj, N, 3))
+    for i in range(n_traj):
+        rs = np.random.default_rng(i)
+        b = rs.normal(scale=1.0, size=(N - 1, 3))
+        R[i, 0] = 0.0
+        R[i, 1:] = np.cumsum(b, axis=0)
+        R[i] -= R[i].mean(axis=0)
+    return np.ascontiguousarray(R)
+
+
+def shear_rate(wi, N=N_BEADS):
+    return wi * np.sin(np.pi / (2.0 * N)) ** 2
+
+
+def kappas(n_traj):
+    K = np.zeros((n_traj, 9))
+    for i in range(n_traj):
+        K[i, 1] = shear_rate(WI[i % len(WI)])
+    return np.ascontiguousarray(K)
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (the batch's own)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def run_gpu(lib, precision, steps, warmup, R0, K, ev):
+    phys = headline_phys()
+    n_traj = R0.shape[0]
+    rng = (A.CRng * n_traj)(*[A.CRng(1, -1, i) for i in range(n_traj)])
+    b = ctypes.c_void_p()
+    A.check(lib, lib.bdsim_batch_create(byref(b), n_traj, N_BEADS, A.dptr(R0), byref(phys),
+                                        A.dptr(K), rng, precision))
+    try:
+        dt = 1e-3
+        if warmup > 0:
+            sim = A.CSim(0.0, (warmup - 1) * dt, dt, 1e-3, 1)
+            A.check(lib, lib.bdsim_batch_run(b, byref(sim)))
+            A.check(lib, lib.bdsim_batch_sync(b))
+        stream = lib.bdsim_batch_stream(b)
+        e0, e1 = ev.create(), ev.create()
+        t_start = warmup * dt
+        sim = A.CSim(t_start, t_start + (steps - 1) * dt, dt, 1e-3, 1)
+        A.check(lib, lib.bdsim_batch_sync(b))
+        w0 = time.perf_counter()
+        ev.record(e0, stream)
+        A.check(lib, lib.bdsim_batch_run(b, byref(sim)))
+        ev.record(e1, stream)
+        A.check(lib, lib.bdsim_batch_sync(b))
+        wall = time.perf_counter() - w0
+        kernel_ms = ev.elapsed_ms(e0, e1)
+        cnt = (A.CCounters * n_traj)()
+        status = np.zeros(n_traj, dtype=np.int32)
+        Rout = np.empty_like(R0)
+        A.check(lib, lib.bdsim_batch_fetch(b, A.dptr(Rout), None, A.iptr(status), cnt))
+    finally:
+        lib.bdsim_batch_destroy(b)
+    mv = np.array([c.matvecs for c in cnt], dtype=np.float64)
+    st = np.array([c.steps for c in cnt])
+    flops = float(np.sum(18.0 * N_BEADS * N_BEADS * mv))
+    return dict(wall_s=wall, kernel_ms=kernel_ms, flops=flops, status_ok=int(np.sum(status == 0)),
+                steps_done=int(st.min()), matvecs_per_step=float(mv.sum() / max(st.sum(), 1)),
+                finite=bool(np.all(np.isfinite(Rout))),
+                lanczos_cold=int(sum(c.lanczos_cold for c in cnt)),
+                fallbacks=int(sum(c.fallbacks for c in cnt)),
+                corrector_per_step=float(sum(c.corrector_iters for c in cnt) / max(st.sum(), 1)))
+
+
+# ---- CPU baseline ----------------------------------------------------------
+
+def _cpu_worker(args):
+    kind, idx, steps = args
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import helpers as H
+    lib = H.ref() if kind == "reference" else H.oracle()
+    phys = headline_phys()
+    R0 = start_configs(max(idx) + 1, N_BEADS) if idx else None
+    t0 = time.perf_counter()
+    for i in idx:
+        phys.flow.kappa0[1] = shear_rate(WI[i % len(WI)])
+        R = np.ascontiguousarray(R0[i])
+        st = A.CRng(1, -1, i)
+        sim = A.CSim(0.0, (steps - 1) * 1e-3, 1e-3, 1e-3, 1)
+        if kind == "reference":
+            lib.ref_integrate(N_BEADS, A.dptr(R), byref(phys), byref(sim), byref(st))
+        else:
+            cnt = A.CCounters()
+            lib.bo_integrate(N_BEADS, A.dptr(R), byref(phys), byref(sim), byref(st), byref(cnt),
+                             None, 0)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(n_traj_sample=192, steps=200):
+    from concurrent.futures import ProcessPoolExecutor
+    kind = "reference" if os.path.exists(os.path.join(REPO, "oracle", "_ref", "libbdref.so")) else "port"
+    if kind == "port" and not os.path.exists(os.path.join(REPO, "oracle", "_build", "liboracle.so")):
+        return None
+    cores = min(os.cpu_count() or 1, 16, n_traj_sample)
+    chunks = [list(range(k, n_traj_sample, cores)) for k in range(cores)]
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=cores) as ex:
+        busy = list(ex.map(_cpu_worker, [(kind, c, steps) for c in chunks]))
+    wall = time.perf_counter() - t0
+    bead_steps = n_traj_sample * N_BEADS * steps
+    return {"value": bead_steps / wall, "unit": "bead-steps/s", "cores": cores, "kind": kind,
+            "sample": (f"{n_traj_sample} of the 4096 chains (same physics, seeds, starts), "
+                       f"{steps} steps each in one integrate call, one process per core; "
+                       f"{sum(busy):.1f} core-seconds of CPU work, wall {wall:.2f} s"
+                       + ("" if kind == "reference" else
+                          "; FP64 C restatement (reference build absent)"))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--traj", type=int, default=N_TRAJ)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="mixed,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(trajectories would shard by index across GPUs; see DESIGN.md). Got --gpus {args.gpus}.",
+              file=sys.stderr)
+        sys.exit(2)
+    lib = A.load_gpu()
+    ev = HipEvents()
+    R0 = start_configs(args.traj, N_BEADS)
+    K = kappas(args.traj)
+    res = {}
+    for form in args.forms.split(","):
+        prec = A.PRECISION_MIXED if form == "mixed" else A.PRECISION_FP64
+        print(f"bench: {form} form, {args.traj} chains, warmup {args.warmup} + {args.steps} steps",
+              file=sys.stderr, flush=True)
+        res[form] = run_gpu(lib, prec, args.steps, args.warmup, R0, K, ev)
+        print(f"bench: {form}: {json.dumps(res[form])}", file=sys.stderr, flush=True)
+    main_form = "mixed" if "mixed" in res else "fp64"
+    r = res[main_form]
+    bead_steps = args.traj * N_BEADS * args.steps
+    value = bead_steps / (r["kernel_ms"] / 1e3)
+    achieved = r["flops"] / (r["kernel_ms"] / 1e3) / 1e12
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "bead-steps/s 4096xN=50 RPY-Chebyshev, mixed vs FP64 GPU; p99 job latency",
+        "value": value,
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["kernel_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_form == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.traj} chains x N=50 FENE b=100, RPY h*=0.3 Chebyshev, "
+                               "SDK EV, shear Wi {0.1,1,10,100}/tau_1 round robin, dt=1e-3, one segment",
+                   "form": main_form, "n_traj": args.traj, "n_beads": N_BEADS},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[main_form],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_form],
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in res and main_form != "fp64":
+        f = res["fp64"]
+        out["fp64"] = {"value": bead_steps / (f["kernel_ms"] / 1e3),
+                       "ms_per_step": f["kernel_ms"] / args.steps,
+                       "achieved_tflops": f["flops"] / (f["kernel_ms"] / 1e3) / 1e12,
+                       "frac_of_78.6": f["flops"] / (f["kernel_ms"] / 1e3) / 1e12 / 78.6}
+        out["mixed_over_fp64"] = out["value"] / out["fp64"]["value"]
+    out["counters"] = {k: {kk: v[kk] for kk in ("matvecs_per_step", "corrector_per_step",
+                                                 "lanczos_cold", "fallbacks", "status_ok",
+                                                 "finite", "wall_s")} for k, v in res.items()}
+    out["p99_job_latency_s"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
