@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -875,8 +876,9 @@ __device__ __forceinline__ void init_counters(Smem<BLOCK, MIXED>& sm) {
     __syncthreads();
 }
 
-template <int BLOCK, bool MIXED>
-__global__ void __launch_bounds__(BLOCK) k_integrate(KArgs a) {
+// WPE: the waves per SIMD the register allocation is held to (1 = unconstrained).
+template <int BLOCK, bool MIXED, int WPE>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) k_integrate(KArgs a) {
     __shared__ Smem<BLOCK, MIXED> sm;
     const int ti = a.list[blockIdx.x];
     const TrajParams* __restrict__ P = a.tp + ti;
@@ -1133,17 +1135,37 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
     return 0;
 }
 
+// Register budget of the integrate kernel for 64-thread workgroups: waves per
+// SIMD the allocation is held to (BDSIM_WPE overrides; 1, 2 or 4).
+int wpe_choice() {
+    static int v = [] {
+        const char* e = std::getenv("BDSIM_WPE");
+        const int w = e ? std::atoi(e) : 2;
+        return (w == 1 || w == 2 || w == 4) ? w : 2;
+    }();
+    return v;
+}
+
+template <bool MIXED>
+void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a) {
+    const int w = wpe_choice();
+    switch (block) {
+        case 64:
+            if (w == 4) hipLaunchKernelGGL((k_integrate<64, MIXED, 4>), grid, dim3(64), 0, stream, a);
+            else if (w == 2) hipLaunchKernelGGL((k_integrate<64, MIXED, 2>), grid, dim3(64), 0, stream, a);
+            else hipLaunchKernelGGL((k_integrate<64, MIXED, 1>), grid, dim3(64), 0, stream, a);
+            break;
+        case 128: hipLaunchKernelGGL((k_integrate<128, MIXED, 1>), grid, dim3(128), 0, stream, a); break;
+        case 256: hipLaunchKernelGGL((k_integrate<256, MIXED, 1>), grid, dim3(256), 0, stream, a); break;
+        case 512: hipLaunchKernelGGL((k_integrate<512, MIXED, 1>), grid, dim3(512), 0, stream, a); break;
+        default: hipLaunchKernelGGL((k_integrate<1024, MIXED, 1>), grid, dim3(1024), 0, stream, a); break;
+    }
+}
+
 template <bool MIXED>
 int launch_bucket(int block, int count, const KArgs& a) {
     if (count == 0) return 0;
-    dim3 grid(count), blk(block);
-    switch (block) {
-        case 64: hipLaunchKernelGGL((k_integrate<64, MIXED>), grid, blk, 0, g.stream, a); break;
-        case 128: hipLaunchKernelGGL((k_integrate<128, MIXED>), grid, blk, 0, g.stream, a); break;
-        case 256: hipLaunchKernelGGL((k_integrate<256, MIXED>), grid, blk, 0, g.stream, a); break;
-        case 512: hipLaunchKernelGGL((k_integrate<512, MIXED>), grid, blk, 0, g.stream, a); break;
-        default: hipLaunchKernelGGL((k_integrate<1024, MIXED>), grid, blk, 0, g.stream, a); break;
-    }
+    launch_integrate<MIXED>(block, dim3(count), g.stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1358,19 +1380,8 @@ int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
     a.t1 = sim->time_end;
     a.dt = sim->dt;
     a.tol = sim->implicit_loop_tol;
-    dim3 grid(b->n_traj), blk(b->block);
-    const bool mixed = b->precision == BDSIM_PRECISION_MIXED;
-#define LAUNCH_B(BS)                                                                              \
-    if (mixed) hipLaunchKernelGGL((k_integrate<BS, true>), grid, blk, 0, b->stream, a);           \
-    else hipLaunchKernelGGL((k_integrate<BS, false>), grid, blk, 0, b->stream, a);
-    switch (b->block) {
-        case 64: LAUNCH_B(64); break;
-        case 128: LAUNCH_B(128); break;
-        case 256: LAUNCH_B(256); break;
-        case 512: LAUNCH_B(512); break;
-        default: LAUNCH_B(1024); break;
-    }
-#undef LAUNCH_B
+    if (b->precision == BDSIM_PRECISION_MIXED) launch_integrate<true>(b->block, dim3(b->n_traj), b->stream, a);
+    else launch_integrate<false>(b->block, dim3(b->n_traj), b->stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
