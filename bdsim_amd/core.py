@@ -374,6 +374,10 @@ def integrate(R, phys, sim, rng, precision=_abi.PRECISION_FP64):
     set to N (the caller's object is left unchanged), `rng` advances in place.
     """
     R = _positions(R)
+    from . import batcher
+    b = batcher.current()
+    if b is not None:          # inside parallel_map(backend="gpu"): batched launch
+        return b.integrate(R, phys, sim, rng)
     N = R.shape[0]
     lib = _abi.load_gpu()
     out = R.copy()

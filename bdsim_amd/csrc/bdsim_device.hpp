@@ -39,6 +39,7 @@ struct TrajParams {
     // excluded volume constants (excluded_volume.cpp:14-34), computed on the host
     double zsbyds5, pt5bydssq, LJa, LJb, SDKa, SDKb, Rmin, Rcutg, Rcutp, RcutpSDK;
     double LJ_at_Rmin, SDK_at_Rmin; // factor values below the soft-core radius
+    double Rmin2, Rcutg2, Rcutp2, RcutpSDK2;   // squared radii (pair loops compare r^2)
     double bend_C;
     double kappa0[9];
     double hstar, mult, fd_tol;
@@ -89,9 +90,23 @@ __host__ __device__ inline WsLayout ws_layout(int N, bool hi) {
 // ---- reductions ----------------------------------------------------------
 // Butterfly sum: every lane ends with the same value (IEEE addition commutes),
 // and the tree is fixed, so the result is deterministic run to run.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// All-lanes sum: quad swaps and row mirrors by DPP, then lane xor 16 / 32.
+// Each stage adds two operands that are the same pair on both partner lanes,
+// so every lane holds bitwise the same total, and the order is fixed.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v += dpp_mov<0xB1>(v);     // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp_mov<0x4E>(v);     // quad_perm [2,3,0,1]: lane ^ 2
+    v += dpp_mov<0x141>(v);    // row_half_mirror: sums of 8
+    v += dpp_mov<0x140>(v);    // row_mirror: sums of 16
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 
@@ -392,6 +407,32 @@ __device__ __forceinline__ void connector_force(const TrajParams& P, const doubl
 }
 
 // ---- excluded volume factor (excluded_volume.cpp:36-58) --------------------
+// From the (clamped) squared distance: the 12-6 powers come from one
+// reciprocal of r^2 and the cutoffs are compared squared. The SDK attraction
+// is evaluated only when an attraction table is present (otherwise it is 0).
+template <typename T>
+__device__ __forceinline__ T ev_factor_sq(const TrajParams& P, T sq, T phi_pair) {
+    if (P.ev_type == 1) return (T)P.zsbyds5 * exp(-sq * (T)P.pt5bydssq);
+    const T inv2 = (T)1 / sq, inv4 = inv2 * inv2, inv8 = inv4 * inv4, inv14 = inv8 * inv4 * inv2;
+    if (P.ev_type == 2) {
+        const T Rcut2 = (T)(P.equil ? P.Rcutg2 : P.Rcutp2);
+        if (sq <= Rcut2 && sq >= (T)P.Rmin2) return (T)P.LJa * inv14 - (T)P.LJb * inv8;
+        if (sq < (T)P.Rmin2) return (T)P.LJ_at_Rmin;
+        return (T)0;
+    }
+    if (P.ev_type == 3) {
+        T fac = (T)0;
+        if (sq <= (T)P.Rcutg2 && sq >= (T)P.Rmin2) fac = (T)P.SDKa * inv14 - (T)P.SDKb * inv8;
+        else if (sq < (T)P.Rmin2) fac = (T)P.SDK_at_Rmin;
+        if (P.has_phi && !P.equil && sq <= (T)P.RcutpSDK2 && sq > (T)P.Rcutg2) {
+            const T SDK_ALPHA = (T)1.530633312, SDK_BETA = (T)1.213115524;
+            fac += SDK_ALPHA * phi_pair * sin(SDK_ALPHA * sq + SDK_BETA);
+        }
+        return fac;
+    }
+    return (T)0;
+}
+
 __device__ __forceinline__ double ev_factor(const TrajParams& P, double r, double phi_pair) {
     if (P.ev_type == 1) return P.zsbyds5 * exp(-r * r * P.pt5bydssq);
     const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4, r14 = r8 * r4 * r2;

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -117,27 +118,39 @@ __device__ inline void tridiag_eigenvector(int m, const double* a, const double*
     }
 }
 
-// Extreme Ritz value of the k x k tridiagonal by 100 Sturm bisections from the
-// Gershgorin bracket (hydrodynamics.cpp:240-262), and its eigenvector; lane 0
-// finds the smallest (count >= 1), lane 1 the largest (count >= k).
-__device__ __noinline__ double ritz_extreme(int k, const double* al, const double* be, int lane,
-                                            double* tri) {
+// Extreme eigenvalues of the k x k Lanczos tridiagonal (hydrodynamics.cpp:240-262):
+// the reference bisects the Gershgorin bracket 100 times on the Sturm count
+// (>= 1 for the smallest, >= k for the largest). Here the 64 lanes of the first
+// wave multisect instead -- lanes 0-31 the smallest, 32-63 the largest -- each
+// round testing 32 interior points and keeping the sub-interval where the count
+// first reaches the target, until the bracket stops shrinking (floating-point
+// resolution); the value is its midpoint, as in the reference. Results:
+// sm_out[0] = smallest, sm_out[1] = largest.
+__device__ __noinline__ void ritz_multisect(int k, const double* al, const double* be,
+                                            double* sm_out) {
+    const int lane = threadIdx.x & 63, half = lane >> 5, j = lane & 31;
     double lo = al[0], hi = al[0];
     for (int i = 0; i < k; ++i) {
         const double rad = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < k - 1 ? fabs(be[i]) : 0.0);
         if (al[i] - rad < lo) lo = al[i] - rad;
         if (hi < al[i] + rad) hi = al[i] + rad;
     }
-    const int target = lane == 0 ? 1 : k;
-    double a1 = lo, b1 = hi;
-    for (int it = 0; it < 100; ++it) {
-        const double mid = 0.5 * (a1 + b1);
-        if (sturm_count(k, al, be, mid) >= target) b1 = mid; else a1 = mid;
+    const int target = half == 0 ? 1 : k;
+    double a = lo, b = hi;
+    for (int round = 0; round < 64; ++round) {
+        const double w = (b - a) / 33.0;
+        const double x = a + (j + 1) * w;
+        const bool ok = sturm_count(k, al, be, x) >= target;
+        const uint64_t m = __ballot(ok);
+        const uint32_t hm = (uint32_t)(half ? (m >> 32) : m);
+        const int f = hm ? __ffs(hm) - 1 : 32;
+        const double nb = f < 32 ? a + (f + 1) * w : b;
+        const double na = f > 0 ? a + f * w : a;
+        const bool stuck = (nb == b && na == a) || !(na < nb);
+        if (!stuck) { a = na; b = nb; }
+        if (__all(stuck)) break;
     }
-    const double r = 0.5 * (a1 + b1);
-    double* rows = tri + lane * 3 * LANCZOS_COLD;
-    tridiag_eigenvector(k, al, be, r, rows, rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
-    return r;
+    if (j == 0) sm_out[half] = 0.5 * (a + b);
 }
 
 // spring.cpp:174-181 out of line: the cubic / rtsafe solvers are large and run
@@ -355,26 +368,29 @@ struct Chain {
 
     // Excluded-volume force (excluded_volume.cpp:60-89) on the own bead at C:
     // F_mu = sum_nu -fac(r) (R_nu - R_mu) over |nu - mu| >= cd.
+    // The mixed form evaluates the pair terms in FP32 (separations formed in
+    // FP64, then rounded) and accumulates them in FP64.
     __device__ void ev_force(const double (*C)[BLOCK], double F[3]) {
         F[0] = F[1] = F[2] = 0.0;
         if (P.ev_type == 0 || P.ev_type == 4 || !own) return;
+        using T = typename std::conditional<MIXED, float, double>::type;
         const double px = C[0][tid], py = C[1][tid], pz = C[2][tid];
         const double* phi = A.blob + P.phi_off;
+        const int cd = P.cd > 1 ? P.cd : 1;
         double fx = 0.0, fy = 0.0, fz = 0.0;
-#pragma unroll 1
+#pragma unroll 2
         for (int nu = 0; nu < N; ++nu) {
             const int sep = nu > tid ? nu - tid : tid - nu;
-            if (sep == 0 || sep < P.cd) continue;
-            const double qx = C[0][nu] - px, qy = C[1][nu] - py, qz = C[2][nu] - pz;
-            double sq = qx * qx + qy * qy + qz * qz;
-            if (sq < 1.0e-12) sq = 1.0e-12;
-            const double r = sqrt(sq);
-            double ph = 0.0;
-            if (P.has_phi) ph = nu > tid ? phi[tid * N + nu] : phi[nu * N + tid];
-            const double fac = ev_factor(P, r, ph);
-            fx += (-fac) * qx;
-            fy += (-fac) * qy;
-            fz += (-fac) * qz;
+            const T qx = (T)(C[0][nu] - px), qy = (T)(C[1][nu] - py), qz = (T)(C[2][nu] - pz);
+            T sq = qx * qx + qy * qy + qz * qz;
+            if (sq < (T)1.0e-12) sq = (T)1.0e-12;
+            T ph = (T)0;
+            if (P.has_phi) ph = (T)(nu > tid ? phi[tid * N + nu] : phi[nu * N + tid]);
+            T fac = ev_factor_sq<T>(P, sq, ph);
+            if (sep < cd) fac = (T)0;
+            fx += (double)((-fac) * qx);
+            fy += (double)((-fac) * qy);
+            fz += (double)((-fac) * qz);
         }
         F[0] = fx; F[1] = fy; F[2] = fz;
     }
@@ -530,10 +546,17 @@ struct Chain {
             return k;
         }
         double* tri = ws + wl.tri;
-        if (tid < 2) sm.Scal[tid] = ritz_extreme(k, sm.Alpha, sm.Beta, tid, tri);
+        if (tid < 64) ritz_multisect(k, sm.Alpha, sm.Beta, sm.Scal);
         __syncthreads();
         rmin = sm.Scal[0];
         rmax = sm.Scal[1];
+        // Ritz vectors of the two extremes by inverse iteration (lanes 0, 1)
+        if (tid < 2) {
+            double* rows = tri + tid * 3 * LANCZOS_COLD;
+            tridiag_eigenvector(k, sm.Alpha, sm.Beta, tid == 0 ? rmin : rmax, rows,
+                                rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
+        }
+        __syncthreads();
         // lift the tridiagonal eigenvectors: v = V^T y (hydrodynamics.cpp:264-273)
         double vmin[3] = {0.0, 0.0, 0.0}, vmax[3] = {0.0, 0.0, 0.0};
         if (own) {
@@ -659,7 +682,8 @@ struct Chain {
         return true;
     }
 
-    __device__ void run(int ti) {
+    // inlined into each kernel so that the kernel's waves-per-EU budget applies
+    __device__ __forceinline__ void run(int ti) {
         double R[3] = {0.0, 0.0, 0.0};
         if (own)
             for (int d = 0; d < 3; ++d) R[d] = A.R[P.R_off + 3 * tid + d];
@@ -1087,6 +1111,10 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
     t.Rcutg = d * std::pow(2.0, 1.0 / 6.0);
     t.LJ_at_Rmin = t.LJa / std::pow(t.Rmin, 14.0) - t.LJb / std::pow(t.Rmin, 8.0);
     t.SDK_at_Rmin = t.SDKa / std::pow(t.Rmin, 14.0) - t.SDKb / std::pow(t.Rmin, 8.0);
+    t.Rmin2 = t.Rmin * t.Rmin;
+    t.Rcutg2 = t.Rcutg * t.Rcutg;
+    t.Rcutp2 = t.Rcutp * t.Rcutp;
+    t.RcutpSDK2 = t.RcutpSDK * t.RcutpSDK;
     t.has_phi = 0;
     t.phi_off = 0;
     if (p.ev_type == BDSIM_EV_SDK && p.n_phi > 0) {
@@ -1136,12 +1164,13 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
 }
 
 // Register budget of the integrate kernel for 64-thread workgroups: waves per
-// SIMD the allocation is held to (BDSIM_WPE overrides; 1, 2 or 4).
+// SIMD the allocation is held to (BDSIM_WPE overrides; 1, 2 or 4; default 4,
+// the fastest measured at 4096 x N=50).
 int wpe_choice() {
     static int v = [] {
         const char* e = std::getenv("BDSIM_WPE");
-        const int w = e ? std::atoi(e) : 2;
-        return (w == 1 || w == 2 || w == 4) ? w : 2;
+        const int w = e ? std::atoi(e) : 4;
+        return (w == 1 || w == 2 || w == 4) ? w : 4;
     }();
     return v;
 }

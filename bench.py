@@ -186,7 +186,7 @@ def _cpu_worker(args):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(n_traj_sample=192, steps=200):
+def cpu_baseline(n_traj_sample=768, steps=400):
     from concurrent.futures import ProcessPoolExecutor
     kind = "reference" if os.path.exists(os.path.join(REPO, "oracle", "_ref", "libbdref.so")) else "port"
     if kind == "port" and not os.path.exists(os.path.join(REPO, "oracle", "_build", "liboracle.so")):
