@@ -43,6 +43,8 @@ struct TrajParams {
     double bend_C;
     double kappa0[9];
     double hstar, mult, fd_tol;
+    double rpy_d[6];                // RPY pair-weight constants (RpyW), FP64 and FP32
+    float rpy_f[6];
     double t0, t1, dt, tol;
     int64_t phi_off, flow_off, ext_off;
     int64_t R_off;                  // doubles into the positions buffer
@@ -504,61 +506,61 @@ __device__ inline void external_at(const TrajParams& P, const ExtEntry* __restri
 }
 
 // ---- RPY mobility (hydrodynamics.cpp:12-54) --------------------------------
-struct RpyConsts {
-    double hstar, inv_hstar, TRPI, TPI, C1, C3, C7, RPI3by4;
-};
-
-__host__ __device__ inline RpyConsts rpy_consts(double hstar) {
-    RpyConsts c;
+// Pair weights of the block D_{mu nu} = om1 I + om2 b b^T (b = R_nu - R_mu)
+// from one reciprocal square root of sq = |b|^2, with the reference's
+// constants folded into five products of h*:
+//   far  (|b| >= 2 sqrt(pi) h*):  om1 = p (1 + C1 h*^2 / sq)
+//                                 om2 = p / sq (1 - 2 pi h*^2 / sq),  p = (3 sqrt(pi)/4) h* / |b|
+//   near (overlapping spheres):   om1 = 1 - (C3 / h*) |b|,  om2 = (C7 / h*) / |b|
+// The far/near test compares sq with (2 sqrt(pi) h*)^2; the weights are
+// continuous where the branches meet (om1 = 7/16, om2 |b|^2 = 3/16), so the
+// choice at the boundary only moves results by rounding. sq is clamped to
+// 1e-12 as in the reference. For the self pair mu == nu, b = 0, so the dyadic
+// term vanishes and the block evaluates to om1(1e-12) I; adding
+// `cdiag` = 1 - om1(1e-12) times the operand restores the identity block.
+// K1..K5 and far2 for h* > 0, in this order.
+__host__ __device__ inline void rpy_folded(double hstar, double k[6]) {
     const double RPI = 1.7724538509055160273;   // sqrt(pi)
-    c.hstar = hstar;
-    c.inv_hstar = hstar > 0.0 ? 1.0 / hstar : 0.0;
-    c.TRPI = 2.0 * RPI;
-    c.TPI = 2.0 * PI;
-    c.C1 = c.TPI / 3.0;
-    c.C3 = 0.75 / RPI * 0.375;
-    c.C7 = 0.09375 / RPI;
-    c.RPI3by4 = RPI * 0.75;
-    return c;
+    const double TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375, C7 = 0.09375 / RPI;
+    k[0] = RPI * 0.75 * hstar;
+    k[1] = C1 * hstar * hstar;
+    k[2] = -TPI * hstar * hstar;
+    k[3] = -C3 / hstar;
+    k[4] = C7 / hstar;
+    k[5] = (2.0 * RPI * hstar) * (2.0 * RPI * hstar);
 }
 
-// Pair weights of the RPY block D_{mu nu} = om1 I + om2 b b^T for separation b
-// (mu != nu); one reciprocal of |b| feeds both branches.
-__device__ __forceinline__ void rpy_weights64(const RpyConsts& c, double sq, double& om1,
-                                              double& om2) {
-    sq = sq < 1.0e-12 ? 1.0e-12 : sq;
-    const double rs = sqrt(sq);
-    const double inv_rs = 1.0 / rs;
-    const double hsbyrs = c.hstar * inv_rs;
-    const double rsbyhs = rs * c.inv_hstar;
-    const double hs2 = hsbyrs * hsbyrs;
-    const double ir2 = inv_rs * inv_rs;
-    const double f1 = c.RPI3by4 * hsbyrs * (1.0 + c.C1 * hs2);
-    const double f2 = c.RPI3by4 * (hsbyrs * ir2) * (1.0 - c.TPI * hs2);
-    const double n1 = 1.0 - c.C3 * rsbyhs;
-    const double n2 = c.C7 * rsbyhs * ir2;
-    const bool far = rsbyhs >= c.TRPI;
-    om1 = far ? f1 : n1;
-    om2 = far ? f2 : n2;
-}
+template <typename T>
+struct RpyW {
+    T K1, K2, K3, K4, K5, far2, clamp, cdiag;
 
-__device__ __forceinline__ void rpy_weights32(float hstar, float inv_hstar, float TRPI, float TPI,
-                                              float C1, float C3, float C7, float RPI3by4,
-                                              float sq, float& om1, float& om2) {
-    sq = sq < 1.0e-12f ? 1.0e-12f : sq;
-    const float inv_rs = __frsqrt_rn(sq);
-    const float rs = sq * inv_rs;
-    const float hsbyrs = hstar * inv_rs;
-    const float rsbyhs = rs * inv_hstar;
-    const float hs2 = hsbyrs * hsbyrs;
-    const float ir2 = inv_rs * inv_rs;
-    const float f1 = RPI3by4 * hsbyrs * (1.0f + C1 * hs2);
-    const float f2 = RPI3by4 * (hsbyrs * ir2) * (1.0f - TPI * hs2);
-    const float n1 = 1.0f - C3 * rsbyhs;
-    const float n2 = C7 * rsbyhs * ir2;
-    const bool far = rsbyhs >= TRPI;
-    om1 = far ? f1 : n1;
-    om2 = far ? f2 : n2;
-}
+    // k: the six rpy_folded constants in the arithmetic type
+    __device__ __forceinline__ void init(const T* k) {
+        K1 = k[0]; K2 = k[1]; K3 = k[2]; K4 = k[3]; K5 = k[4]; far2 = k[5];
+        clamp = (T)1.0e-12;
+        T om1, om2;
+        weights((T)0, om1, om2);
+        cdiag = (T)1 - om1;
+    }
+
+    __device__ __forceinline__ static T rsq(T x) {
+        if constexpr (sizeof(T) == 4) return __frsqrt_rn(x);
+        else return rsqrt(x);
+    }
+
+    __device__ __forceinline__ void weights(T sq, T& om1, T& om2) const {
+        sq = fmax(sq, clamp);
+        const T ir = rsq(sq);
+        const T ir2 = ir * ir;
+        const T p = K1 * ir;
+        const T om1f = p * fma(K2, ir2, (T)1);
+        const T om2f = (p * ir2) * fma(K3, ir2, (T)1);
+        const T om1n = fma(K4, sq * ir, (T)1);
+        const T om2n = K5 * ir;
+        const bool far = sq >= far2;
+        om1 = far ? om1f : om1n;
+        om2 = far ? om2f : om2n;
+    }
+};
 
 }  // namespace bdgpu

@@ -211,7 +211,10 @@ struct Chain {
         }
     }
 
-    // y = D(sm.R) v   (Diffusion::apply, mobility.cpp:53-56)
+    // y = D(sm.R) v   (Diffusion::apply, mobility.cpp:53-56). Each lane runs
+    // over all N partners with its own bead included: the self pair evaluates
+    // to om1(1e-12) I and cdiag * v restores the identity block (RpyW), so the
+    // pair loop has no lane-dependent select.
     __device__ void apply(const double v[3], double y[3]) {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
         __syncthreads();
@@ -225,61 +228,59 @@ struct Chain {
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (own) {
             if constexpr (!MIXED) {
-                const double hs = P.hstar, ihs = 1.0 / P.hstar;
-                const double RPI = 1.7724538509055160273;
-                const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0,
-                             C3 = 0.75 / RPI * 0.375, C7 = 0.09375 / RPI, R34 = RPI * 0.75;
+                RpyW<double> W;
+                W.init(P.rpy_d);
                 const double rx = sm.R[0][tid], ry = sm.R[1][tid], rz = sm.R[2][tid];
 #pragma unroll 2
                 for (int nu = 0; nu < N; ++nu) {
                     const double bx = sm.R[0][nu] - rx, by = sm.R[1][nu] - ry, bz = sm.R[2][nu] - rz;
-                    double sq = bx * bx + by * by + bz * bz;
-                    sq = sq < 1.0e-12 ? 1.0e-12 : sq;
-                    // RPY pair weights (hydrodynamics.cpp:88-98) from one rsqrt
-                    const double inv_rs = rsqrt(sq);
-                    const double rs = sq * inv_rs;
-                    const double hsbyrs = hs * inv_rs, rsbyhs = rs * ihs;
-                    const double hs2 = hsbyrs * hsbyrs, ir2 = inv_rs * inv_rs;
-                    const bool far = rsbyhs >= TRPI;
-                    double om1 = far ? R34 * hsbyrs * (1.0 + C1 * hs2) : 1.0 - C3 * rsbyhs;
-                    double om2 = far ? R34 * (hsbyrs * ir2) * (1.0 - TPI * hs2) : C7 * rsbyhs * ir2;
-                    if (nu == tid) { om1 = 1.0; om2 = 0.0; }
+                    double om1, om2;
+                    W.weights(bx * bx + by * by + bz * bz, om1, om2);
                     const double xx = sm.X[0][nu], xy = sm.X[1][nu], xz = sm.X[2][nu];
                     const double t = om2 * (bx * xx + by * xy + bz * xz);
-                    ax += om1 * xx + t * bx;
-                    ay += om1 * xy + t * by;
-                    az += om1 * xz + t * bz;
+                    ax = fma(om1, xx, ax); ax = fma(t, bx, ax);
+                    ay = fma(om1, xy, ay); ay = fma(t, by, ay);
+                    az = fma(om1, xz, az); az = fma(t, bz, az);
                 }
+                ax = fma(W.cdiag, v[0], ax);
+                ay = fma(W.cdiag, v[1], ay);
+                az = fma(W.cdiag, v[2], az);
             } else {
-                const float hs = (float)P.hstar, ihs = (float)(1.0 / P.hstar);
-                const float RPI = 1.7724538509055160273f;
-                const float TRPI = 2.0f * RPI, TPI = (float)(2.0 * PI), C1 = (float)(2.0 * PI / 3.0),
-                            C3 = 0.75f / RPI * 0.375f, C7 = 0.09375f / RPI, R34 = RPI * 0.75f;
+                RpyW<float> W;
+                W.init(P.rpy_f);
                 const float rx = sm.Rf[0][tid], ry = sm.Rf[1][tid], rz = sm.Rf[2][tid];
+                // FP32 pair terms summed over runs of 8 partners, then
+                // accumulated in FP64
+                auto pair = [&](int nu, float& fx, float& fy, float& fz) {
+                    const float bx = sm.Rf[0][nu] - rx, by = sm.Rf[1][nu] - ry, bz = sm.Rf[2][nu] - rz;
+                    float om1, om2;
+                    W.weights(bx * bx + by * by + bz * bz, om1, om2);
+                    const float xx = sm.Xf[0][nu], xy = sm.Xf[1][nu], xz = sm.Xf[2][nu];
+                    const float t = om2 * (bx * xx + by * xy + bz * xz);
+                    fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
+                    fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);
+                    fz = fmaf(om1, xz, fz); fz = fmaf(t, bz, fz);
+                };
                 int nu = 0;
-                while (nu < N) {
-                    // FP32 pair terms summed over runs of 8 partners, then
-                    // accumulated in FP64
-                    const int end = min(nu + 8, N);
+                for (; nu + 8 <= N; nu += 8) {
                     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-#pragma unroll 2
-                    for (; nu < end; ++nu) {
-                        const float bx = sm.Rf[0][nu] - rx, by = sm.Rf[1][nu] - ry,
-                                    bz = sm.Rf[2][nu] - rz;
-                        float om1, om2;
-                        rpy_weights32(hs, ihs, TRPI, TPI, C1, C3, C7, R34,
-                                      bx * bx + by * by + bz * bz, om1, om2);
-                        if (nu == tid) { om1 = 1.0f; om2 = 0.0f; }
-                        const float xx = sm.Xf[0][nu], xy = sm.Xf[1][nu], xz = sm.Xf[2][nu];
-                        const float t = om2 * (bx * xx + by * xy + bz * xz);
-                        fx += om1 * xx + t * bx;
-                        fy += om1 * xy + t * by;
-                        fz += om1 * xz + t * bz;
-                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) pair(nu + k, fx, fy, fz);
                     ax += (double)fx;
                     ay += (double)fy;
                     az += (double)fz;
                 }
+                if (nu < N) {
+                    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                    for (; nu < N; ++nu) pair(nu, fx, fy, fz);
+                    ax += (double)fx;
+                    ay += (double)fy;
+                    az += (double)fz;
+                }
+                const double cd = (double)W.cdiag;
+                ax = fma(cd, v[0], ax);
+                ay = fma(cd, v[1], ay);
+                az = fma(cd, v[2], az);
             }
         }
         y[0] = ax; y[1] = ay; y[2] = az;
@@ -1082,6 +1083,14 @@ int block_for(int N) {
     return 1024;
 }
 
+// h* and the folded RPY pair-weight constants (zero without HI).
+void set_rpy(TrajParams& t, double hstar) {
+    t.hstar = hstar;
+    double k[6] = {0, 0, 0, 0, 0, 0};
+    if (hstar > 0.0) rpy_folded(hstar, k);
+    for (int j = 0; j < 6; ++j) { t.rpy_d[j] = k[j]; t.rpy_f[j] = (float)k[j]; }
+}
+
 // Host packing of one trajectory (EV constants exactly as
 // excluded_volume.cpp:14-34, evaluated with the host libm).
 int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double>& blob,
@@ -1111,6 +1120,7 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
     t.Rcutg = d * std::pow(2.0, 1.0 / 6.0);
     t.LJ_at_Rmin = t.LJa / std::pow(t.Rmin, 14.0) - t.LJb / std::pow(t.Rmin, 8.0);
     t.SDK_at_Rmin = t.SDKa / std::pow(t.Rmin, 14.0) - t.SDKb / std::pow(t.Rmin, 8.0);
+    set_rpy(t, p.hstar);
     t.Rmin2 = t.Rmin * t.Rmin;
     t.Rcutg2 = t.Rcutg * t.Rcutg;
     t.Rcutp2 = t.Rcutp * t.Rcutp;
@@ -1149,7 +1159,6 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
         }
         ext.push_back(x);
     }
-    t.hstar = p.hstar;
     t.hi_method = p.hi_method;
     t.mult = p.ncheb_multiplier;
     t.fd_tol = p.fd_err_max;
@@ -1587,7 +1596,7 @@ int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision, c
     TrajParams tp;
     std::memset(&tp, 0, sizeof tp);
     tp.N = nbeads;
-    tp.hstar = hstar;
+    set_rpy(tp, hstar);
     const size_t nd = (size_t)n_traj * 3 * nbeads;
     if (int rc = g.tp.ensure(sizeof(TrajParams))) return rc;
     if (int rc = g.R.ensure(sizeof(double) * nd)) return rc;
