@@ -38,7 +38,6 @@ struct TrajParams {
     double sqrtb, q0, natural_length;
     // excluded volume constants (excluded_volume.cpp:14-34), computed on the host
     double zsbyds5, pt5bydssq, LJa, LJb, SDKa, SDKb, Rmin, Rcutg, Rcutp, RcutpSDK;
-    double LJ_at_Rmin, SDK_at_Rmin; // factor values below the soft-core radius
     double Rmin2, Rcutg2, Rcutp2, RcutpSDK2;   // squared radii (pair loops compare r^2)
     double bend_C;
     double kappa0[9];
@@ -408,53 +407,25 @@ __device__ __forceinline__ void connector_force(const TrajParams& P, const doubl
     F[0] = f * b[0]; F[1] = f * b[1]; F[2] = f * b[2];
 }
 
-// ---- excluded volume factor (excluded_volume.cpp:36-58) --------------------
-// From the (clamped) squared distance: the 12-6 powers come from one
-// reciprocal of r^2 and the cutoffs are compared squared. The SDK attraction
-// is evaluated only when an attraction table is present (otherwise it is 0).
+// ---- LDS-qualified pointers ----------------------------------------------------
+// A pointer into the workgroup's shared arrays, typed so that reads through it
+// are LDS instructions (ds_read) wherever the access sits after inlining.
 template <typename T>
-__device__ __forceinline__ T ev_factor_sq(const TrajParams& P, T sq, T phi_pair) {
-    if (P.ev_type == 1) return (T)P.zsbyds5 * exp(-sq * (T)P.pt5bydssq);
-    const T inv2 = (T)1 / sq, inv4 = inv2 * inv2, inv8 = inv4 * inv4, inv14 = inv8 * inv4 * inv2;
-    if (P.ev_type == 2) {
-        const T Rcut2 = (T)(P.equil ? P.Rcutg2 : P.Rcutp2);
-        if (sq <= Rcut2 && sq >= (T)P.Rmin2) return (T)P.LJa * inv14 - (T)P.LJb * inv8;
-        if (sq < (T)P.Rmin2) return (T)P.LJ_at_Rmin;
-        return (T)0;
-    }
-    if (P.ev_type == 3) {
-        T fac = (T)0;
-        if (sq <= (T)P.Rcutg2 && sq >= (T)P.Rmin2) fac = (T)P.SDKa * inv14 - (T)P.SDKb * inv8;
-        else if (sq < (T)P.Rmin2) fac = (T)P.SDK_at_Rmin;
-        if (P.has_phi && !P.equil && sq <= (T)P.RcutpSDK2 && sq > (T)P.Rcutg2) {
-            const T SDK_ALPHA = (T)1.530633312, SDK_BETA = (T)1.213115524;
-            fac += SDK_ALPHA * phi_pair * sin(SDK_ALPHA * sq + SDK_BETA);
-        }
-        return fac;
-    }
-    return (T)0;
-}
+using lds_cptr = const __attribute__((address_space(3))) T*;
+template <typename T>
+__device__ __forceinline__ lds_cptr<T> as_lds(const T* p) { return (lds_cptr<T>)p; }
 
-__device__ __forceinline__ double ev_factor(const TrajParams& P, double r, double phi_pair) {
-    if (P.ev_type == 1) return P.zsbyds5 * exp(-r * r * P.pt5bydssq);
-    const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4, r14 = r8 * r4 * r2;
-    if (P.ev_type == 2) {
-        const double Rcut = P.equil ? P.Rcutg : P.Rcutp;
-        if (r <= Rcut && r >= P.Rmin) return P.LJa / r14 - P.LJb / r8;
-        if (r < P.Rmin) return P.LJ_at_Rmin;
-        return 0.0;
-    }
-    if (P.ev_type == 3) {
-        double fac = 0.0;
-        if (r <= P.Rcutg && r >= P.Rmin) fac = P.SDKa / r14 - P.SDKb / r8;
-        else if (r < P.Rmin) fac = P.SDK_at_Rmin;
-        if (!P.equil && r <= P.RcutpSDK && r > P.Rcutg) {
-            const double SDK_ALPHA = 1.530633312, SDK_BETA = 1.213115524;
-            fac += SDK_ALPHA * phi_pair * sin(SDK_ALPHA * r * r + SDK_BETA);
-        }
-        return fac;
-    }
-    return 0.0;
+// ---- excluded volume (excluded_volume.cpp:36-58) ----------------------------
+// Reciprocal of x > 0: v_rcp_f32 (1 ulp) in FP32; in FP64 v_rcp_f64 refined by
+// two Newton steps (e = 1 - x r, r += r e), which brings it to within an ulp
+// of 1/x.
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_fast(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 // ---- flow and external forces (flow.cpp:5-26, external_force.cpp:8-52) ------

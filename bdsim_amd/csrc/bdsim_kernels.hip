@@ -230,13 +230,15 @@ struct Chain {
             if constexpr (!MIXED) {
                 RpyW<double> W;
                 W.init(P.rpy_d);
-                const double rx = sm.R[0][tid], ry = sm.R[1][tid], rz = sm.R[2][tid];
+                const lds_cptr<double> r0 = as_lds(sm.R[0]), r1 = as_lds(sm.R[1]), r2 = as_lds(sm.R[2]);
+                const lds_cptr<double> v0 = as_lds(sm.X[0]), v1 = as_lds(sm.X[1]), v2 = as_lds(sm.X[2]);
+                const double rx = r0[tid], ry = r1[tid], rz = r2[tid];
 #pragma unroll 2
                 for (int nu = 0; nu < N; ++nu) {
-                    const double bx = sm.R[0][nu] - rx, by = sm.R[1][nu] - ry, bz = sm.R[2][nu] - rz;
+                    const double bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
                     double om1, om2;
                     W.weights(bx * bx + by * by + bz * bz, om1, om2);
-                    const double xx = sm.X[0][nu], xy = sm.X[1][nu], xz = sm.X[2][nu];
+                    const double xx = v0[nu], xy = v1[nu], xz = v2[nu];
                     const double t = om2 * (bx * xx + by * xy + bz * xz);
                     ax = fma(om1, xx, ax); ax = fma(t, bx, ax);
                     ay = fma(om1, xy, ay); ay = fma(t, by, ay);
@@ -248,14 +250,16 @@ struct Chain {
             } else {
                 RpyW<float> W;
                 W.init(P.rpy_f);
-                const float rx = sm.Rf[0][tid], ry = sm.Rf[1][tid], rz = sm.Rf[2][tid];
+                const lds_cptr<float> r0 = as_lds(sm.Rf[0]), r1 = as_lds(sm.Rf[1]), r2 = as_lds(sm.Rf[2]);
+                const lds_cptr<float> v0 = as_lds(sm.Xf[0]), v1 = as_lds(sm.Xf[1]), v2 = as_lds(sm.Xf[2]);
+                const float rx = r0[tid], ry = r1[tid], rz = r2[tid];
                 // FP32 pair terms summed over runs of 8 partners, then
                 // accumulated in FP64
                 auto pair = [&](int nu, float& fx, float& fy, float& fz) {
-                    const float bx = sm.Rf[0][nu] - rx, by = sm.Rf[1][nu] - ry, bz = sm.Rf[2][nu] - rz;
+                    const float bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
                     float om1, om2;
                     W.weights(bx * bx + by * by + bz * bz, om1, om2);
-                    const float xx = sm.Xf[0][nu], xy = sm.Xf[1][nu], xz = sm.Xf[2][nu];
+                    const float xx = v0[nu], xy = v1[nu], xz = v2[nu];
                     const float t = om2 * (bx * xx + by * xy + bz * xz);
                     fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
                     fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);
@@ -367,33 +371,112 @@ struct Chain {
         for (int d = 0; d < 3; ++d) F[d] = Cst * (A_[d] + B_[d] + C_[d] + D_[d]);
     }
 
-    // Excluded-volume force (excluded_volume.cpp:60-89) on the own bead at C:
-    // F_mu = sum_nu -fac(r) (R_nu - R_mu) over |nu - mu| >= cd.
-    // The mixed form evaluates the pair terms in FP32 (separations formed in
-    // FP64, then rounded) and accumulates them in FP64.
+    // Excluded-volume force on this lane's bead (excluded_volume.cpp:60-89) at
+    // configuration C (LDS, [3][BLOCK]): the sum over partners nu with
+    // |nu - mu| >= cd of -fac(|q|^2) q, q = C_nu - C_mu. The potential type,
+    // its constants and the contour exclusion are resolved before the pair
+    // loop. With cd <= 1 only the self pair is excluded, and it needs no test:
+    // q = 0 there, so its term vanishes whatever the (finite) factor. The
+    // mixed form works on FP32 positions relative to bead 0, with FP32 pair
+    // terms summed over runs of 8 partners and accumulated in FP64.
     __device__ void ev_force(const double (*C)[BLOCK], double F[3]) {
         F[0] = F[1] = F[2] = 0.0;
-        if (P.ev_type == 0 || P.ev_type == 4 || !own) return;
-        using T = typename std::conditional<MIXED, float, double>::type;
-        const double px = C[0][tid], py = C[1][tid], pz = C[2][tid];
-        const double* phi = A.blob + P.phi_off;
-        const int cd = P.cd > 1 ? P.cd : 1;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-#pragma unroll 2
-        for (int nu = 0; nu < N; ++nu) {
-            const int sep = nu > tid ? nu - tid : tid - nu;
-            const T qx = (T)(C[0][nu] - px), qy = (T)(C[1][nu] - py), qz = (T)(C[2][nu] - pz);
-            T sq = qx * qx + qy * qy + qz * qz;
-            if (sq < (T)1.0e-12) sq = (T)1.0e-12;
-            T ph = (T)0;
-            if (P.has_phi) ph = (T)(nu > tid ? phi[tid * N + nu] : phi[nu * N + tid]);
-            T fac = ev_factor_sq<T>(P, sq, ph);
-            if (sep < cd) fac = (T)0;
-            fx += (double)((-fac) * qx);
-            fy += (double)((-fac) * qy);
-            fz += (double)((-fac) * qz);
+        const int et = P.ev_type;
+        if (et != 1 && et != 2 && et != 3) return;
+        if constexpr (MIXED) {
+            __syncthreads();
+            for (int d = 0; d < 3; ++d) sm.Xf[d][tid] = own ? (float)(C[d][tid] - C[d][0]) : 0.0f;
+            __syncthreads();
         }
-        F[0] = fx; F[1] = fy; F[2] = fz;
+        if (!own) return;
+        // The self pair takes the factor's value at Rmin, finite in FP32 for
+        // Rmin >= 0.1; a smaller soft-core radius keeps the explicit exclusion
+        // test, which discards that value.
+        const bool cdt = P.cd > 1 || (et != 1 && !(P.Rmin2 >= 1.0e-2));
+        if (et == 1) {
+            if (cdt) ev_loop<1, false, true>(C, F); else ev_loop<1, false, false>(C, F);
+        } else if (et == 2) {
+            if (cdt) ev_loop<2, false, true>(C, F); else ev_loop<2, false, false>(C, F);
+        } else if (P.has_phi && !P.equil) {
+            ev_loop<3, true, true>(C, F);
+        } else {
+            if (cdt) ev_loop<3, false, true>(C, F); else ev_loop<3, false, false>(C, F);
+        }
+    }
+
+    // The pair factor (excluded_volume.cpp:36-58) from the clamped r^2: Gauss
+    // z*/d*^5 exp(-r^2 / 2d*^2); LJ and SDK a/r^14 - b/r^8 = (a/r^6 - b)/r^8
+    // between Rmin and the cutoff (compared squared) and its value at Rmin
+    // below, i.e. the same expression at max(r^2, Rmin^2), so there is one
+    // select and no branch; with an attraction table the SDK tail
+    // alpha phi sin(alpha r^2 + beta) between the cutoffs.
+    template <int EVT, bool TAIL, bool CDT>
+    __device__ __forceinline__ void ev_loop(const double (*C)[BLOCK], double F[3]) {
+        using T = typename std::conditional<MIXED, float, double>::type;
+        const T g_a = (T)P.zsbyds5, g_b = (T)(-P.pt5bydssq);
+        const T ca = (T)(EVT == 2 ? P.LJa : P.SDKa), cb = (T)(EVT == 2 ? P.LJb : P.SDKb);
+        const T rmin2 = (T)P.Rmin2;
+        const T rcut2 = (T)(EVT == 2 ? (P.equil ? P.Rcutg2 : P.Rcutp2) : P.Rcutg2);
+        // below Rmin the factor is constant even where Rmin exceeds the cutoff
+        const T rcut2e = fmax(rcut2, rmin2);
+        const T rtail2 = (T)P.RcutpSDK2;
+        const double* phi = A.blob + P.phi_off;
+        const int cd = P.cd;
+        lds_cptr<T> x0, x1, x2;
+        if constexpr (MIXED) {
+            x0 = as_lds(sm.Xf[0]); x1 = as_lds(sm.Xf[1]); x2 = as_lds(sm.Xf[2]);
+        } else {
+            x0 = as_lds(C[0]); x1 = as_lds(C[1]); x2 = as_lds(C[2]);
+        }
+        const T px = x0[tid], py = x1[tid], pz = x2[tid];
+        auto pair = [&](int nu, T& fx, T& fy, T& fz) {
+            const T qx = x0[nu] - px, qy = x1[nu] - py, qz = x2[nu] - pz;
+            const T sq = fmax(qx * qx + qy * qy + qz * qz, (T)1.0e-12);
+            T fac;
+            if constexpr (EVT == 1) {
+                fac = g_a * exp(g_b * sq);
+            } else {
+                const T inv2 = rcp_fast(fmax(sq, rmin2));
+                const T inv4 = inv2 * inv2, inv8 = inv4 * inv4;
+                const T core = fma(ca * inv4, inv2, -cb) * inv8;
+                fac = sq <= rcut2e ? core : (T)0;
+                if constexpr (TAIL) {
+                    if (sq <= rtail2 && sq > rcut2) {
+                        const T SDK_ALPHA = (T)1.530633312, SDK_BETA = (T)1.213115524;
+                        const T ph = (T)(nu > tid ? phi[tid * N + nu] : phi[nu * N + tid]);
+                        fac += SDK_ALPHA * ph * sin(SDK_ALPHA * sq + SDK_BETA);
+                    }
+                }
+            }
+            if constexpr (CDT) {
+                const int sep = nu > tid ? nu - tid : tid - nu;
+                if (sep < cd) fac = (T)0;
+            }
+            fx = fma(-fac, qx, fx);
+            fy = fma(-fac, qy, fy);
+            fz = fma(-fac, qz, fz);
+        };
+        if constexpr (MIXED) {
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            int nu = 0;
+            for (; nu + 8 <= N; nu += 8) {
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) pair(nu + k, fx, fy, fz);
+                ax += (double)fx; ay += (double)fy; az += (double)fz;
+            }
+            if (nu < N) {
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                for (; nu < N; ++nu) pair(nu, fx, fy, fz);
+                ax += (double)fx; ay += (double)fy; az += (double)fz;
+            }
+            F[0] = ax; F[1] = ay; F[2] = az;
+        } else {
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll 2
+            for (int nu = 0; nu < N; ++nu) pair(nu, fx, fy, fz);
+            F[0] = fx; F[1] = fy; F[2] = fz;
+        }
     }
 
     // non-spring + external force at C and time t (model.hpp:56-64)
@@ -1118,8 +1201,6 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
     t.Rcutp = p.max_cutoff;
     t.RcutpSDK = p.max_cutoff;
     t.Rcutg = d * std::pow(2.0, 1.0 / 6.0);
-    t.LJ_at_Rmin = t.LJa / std::pow(t.Rmin, 14.0) - t.LJb / std::pow(t.Rmin, 8.0);
-    t.SDK_at_Rmin = t.SDKa / std::pow(t.Rmin, 14.0) - t.SDKb / std::pow(t.Rmin, 8.0);
     set_rpy(t, p.hstar);
     t.Rmin2 = t.Rmin * t.Rmin;
     t.Rcutg2 = t.Rcutg * t.Rcutg;

@@ -217,6 +217,90 @@ def test_mixed_precision_within_1e5(gpu, orc):
         assert rel(o, r) < 1e-5, rel(o, r)
 
 
+def test_mixed_form_every_golden_case(gpu, golden):
+    """Every golden trajectory in the mixed form (FP32 pair terms for D x and
+    excluded volume, FP64 elsewhere), one batched launch per segment, against
+    the reference's FP64 positions. Over these 10-100 step segments the mixed
+    form stays within 1e-5 relative (the largest error, N=50 SDK at Wi=10, is
+    below 1e-6)."""
+    cases = [c for c, _ in _golden_items(golden)]
+    keep = [H.Packed(c["phys"], c["N"]) for c in cases]
+    Rs = [golden[f"traj/{c['name']}/R0"] for c in cases]
+    sts = [H.rng_of(c["seed"]) for c in cases]
+    worst = {}
+    for seg in range(max(c["segments"] for c in cases)):
+        idx = [i for i, c in enumerate(cases) if seg < c["segments"]]
+        items = [(cases[i]["N"], Rs[i], keep[i].c, H.csim(cases[i]["sim"]), sts[i]) for i in idx]
+        out, rngs, status, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+        for j, i in enumerate(idx):
+            name = cases[i]["name"]
+            assert status[j] == golden[f"traj/{name}/rc"][seg], name
+            sts[i] = rngs[j]
+            if status[j] != 0:
+                continue
+            worst[name] = max(worst.get(name, 0.0), rel(out[j], golden[f"traj/{name}/R"][seg]))
+            Rs[i] = out[j]
+    for c, st in zip(cases, sts):
+        assert (st.ix, st.iy, st.seed) == tuple(golden[f"traj/{c['name']}/rng"]), c["name"]
+    print("mixed-form relative error per case:",
+          {k: f"{v:.1e}" for k, v in sorted(worst.items(), key=lambda kv: -kv[1])})
+    for k, v in worst.items():
+        assert v < 1e-5, (k, v)
+
+
+def _helix(N, seed):
+    """A tight helix (radius 0.6, 100 degrees and 0.25 rise per bead): bonds
+    near 0.95, beads 3 and 4 apart at about 1.0, 2 apart at 1.3."""
+    rs = np.random.default_rng(seed)
+    k = np.arange(N)
+    turn = np.deg2rad(100.0)
+    R = np.stack([0.6 * np.cos(turn * k), 0.6 * np.sin(turn * k), 0.25 * k], axis=1)
+    return np.ascontiguousarray(R + 0.02 * rs.normal(size=(N, 3)))
+
+
+def _phi_table(N, seed):
+    rs = np.random.default_rng(seed)
+    a = rs.uniform(0.0, 1.0, size=(N, N))
+    return (a + a.T) / 2
+
+
+EV_VARIANTS = {
+    "gauss": dict(ev=1, zstar=1.0, dstar=0.5),
+    "lj_cd3": dict(ev=2, zstar=0.3, dstar=0.8, cd=3),
+    "lj_soft_core_0": dict(ev=2, zstar=0.1, dstar=0.7, min_cutoff=0.0),
+    "sdk_soft_core_0": dict(ev=3, dstar=0.95, min_cutoff=0.0),
+    "sdk_phi_cd2": dict(ev=3, dstar=0.75, cd=2, phi="table"),
+    "sdk_phi_equil": dict(ev=3, dstar=0.9, equil=1, phi="table"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EV_VARIANTS))
+def test_ev_variants_both_forms(gpu, orc, name):
+    """Excluded-volume variants beyond the golden set (zero soft-core radius,
+    contour exclusion with an attraction table): the FP64 force matches the
+    oracle to 1e-12; 100 HI steps match it to 1e-10 (FP64) and 1e-5 (mixed)."""
+    N = 24
+    kw = dict(EV_VARIANTS[name])
+    if kw.get("phi") == "table":
+        kw["phi"] = _phi_table(N, 5)
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.3, kappa=C.wi_kappa(1.0), **kw)
+    p = H.Packed(d, N)
+    R0 = _helix(N, 9)
+    F = np.empty_like(R0)
+    A.check(gpu, gpu.bdsim_ev_force_batch(1, N, A.dptr(R0), byref(p.c), A.dptr(F)))
+    Fo = np.empty_like(R0)
+    orc.bo_ev_force(N, A.dptr(R0), byref(p.c), A.dptr(Fo))
+    assert np.abs(Fo).max() > 0.0
+    assert rel(F, Fo) < 1e-12
+    sim = H.csim(C.sim_dict(0.0, 0.099, 1e-3, 1e-6))
+    want, rc, _, _ = H.oracle_integrate(orc, N, R0, p.c, sim, H.rng_of(3))
+    assert rc == 0
+    for prec, tol in ((A.PRECISION_FP64, 1e-10), (A.PRECISION_MIXED, 1e-5)):
+        out, _, status, _, _ = H.gpu_batch(gpu, [(N, R0, p.c, sim, H.rng_of(3))], precision=prec)
+        assert status[0] == 0
+        assert rel(out[0], want) < tol, (prec, rel(out[0], want))
+
+
 def test_device_resident_batch_matches(gpu):
     """bdsim_batch_* (positions and RNG resident in HBM) reproduces
     bdsim_integrate_batch over two consecutive segments."""
