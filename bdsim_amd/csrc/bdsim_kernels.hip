@@ -53,15 +53,19 @@ struct KArgs {
 template <int BLOCK, bool MIXED>
 struct Smem {
     static constexpr int NW = BLOCK / 64;
-    static constexpr int FB = MIXED ? BLOCK : 1;   // FP32 arrays (mixed form)
+    // single-wave chains walk bead pairs (mu, mu + k) on doubled FP32 copies
+    static constexpr bool PAIRS = MIXED && BLOCK == 64;
+    static constexpr int FB = MIXED ? (PAIRS ? 2 * BLOCK : BLOCK) : 1;
     static constexpr int DB = MIXED ? 1 : BLOCK;   // FP64 operand (FP64 form)
     double R[3][BLOCK];         // configuration the mobility is evaluated at
     double P[3][BLOCK];         // force configuration; corrector positions
     double X[3][DB];            // mat-vec operand
     float Rf[3][FB];            // mixed form: positions relative to bead 0
-    float Xf[3][FB];            // mixed form: mat-vec operand
     double B[4][BLOCK];         // bond-level arrays (unit bonds + lengths, bonds)
-    double V[3][BLOCK];         // neighbour exchange; triangular-product operand
+    union {
+        double V[3][BLOCK];     // neighbour exchange; triangular-product operand
+        float W[3][2 * BLOCK];  // mixed form: mat-vec operand, EV positions
+    };
     double Red[2 * NW];
     double Alpha[LANCZOS_COLD];
     double Beta[LANCZOS_COLD];
@@ -167,6 +171,7 @@ __device__ __noinline__ int solve_connector_ool(double g0, double g1, double g2,
 template <int BLOCK, bool MIXED>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
+    using S = Smem<BLOCK, MIXED>;
     Smem<BLOCK, MIXED>& sm;
     const TrajParams& P;        // global, read with scalar loads
     const KArgs& A;
@@ -205,8 +210,12 @@ struct Chain {
         for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
         __syncthreads();
         if constexpr (MIXED) {
-            for (int d = 0; d < 3; ++d)
-                sm.Rf[d][tid] = own ? (float)(sm.R[d][tid] - sm.R[d][0]) : 0.0f;
+            if (own)
+                for (int d = 0; d < 3; ++d) {
+                    const float r = (float)(sm.R[d][tid] - sm.R[d][0]);
+                    sm.Rf[d][tid] = r;
+                    if constexpr (S::PAIRS) sm.Rf[d][tid + N] = r;
+                }
             __syncthreads();
         }
     }
@@ -219,12 +228,20 @@ struct Chain {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
         __syncthreads();
         if constexpr (MIXED) {
-            for (int d = 0; d < 3; ++d) sm.Xf[d][tid] = (float)v[d];
+            if (own)
+                for (int d = 0; d < 3; ++d) {
+                    sm.W[d][tid] = (float)v[d];
+                    if constexpr (S::PAIRS) sm.W[d][tid + N] = (float)v[d];
+                }
         } else {
             for (int d = 0; d < 3; ++d) sm.X[d][tid] = v[d];
         }
         __syncthreads();
         bump(sm.cnt.matvecs);
+        if constexpr (S::PAIRS) {
+            apply_pairs(v, y);
+            return;
+        }
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (own) {
             if constexpr (!MIXED) {
@@ -251,7 +268,7 @@ struct Chain {
                 RpyW<float> W;
                 W.init(P.rpy_f);
                 const lds_cptr<float> r0 = as_lds(sm.Rf[0]), r1 = as_lds(sm.Rf[1]), r2 = as_lds(sm.Rf[2]);
-                const lds_cptr<float> v0 = as_lds(sm.Xf[0]), v1 = as_lds(sm.Xf[1]), v2 = as_lds(sm.Xf[2]);
+                const lds_cptr<float> v0 = as_lds(sm.W[0]), v1 = as_lds(sm.W[1]), v2 = as_lds(sm.W[2]);
                 const float rx = r0[tid], ry = r1[tid], rz = r2[tid];
                 // FP32 pair terms summed over runs of 8 partners, then
                 // accumulated in FP64
@@ -288,6 +305,88 @@ struct Chain {
             }
         }
         y[0] = ax; y[1] = ay; y[2] = az;
+    }
+
+    // D x for a single-wave chain in the mixed form, each unordered bead pair
+    // evaluated once. Lane mu visits partners nu = mu + k (mod N, read from
+    // the doubled copies) for k = K..1, K = floor((N-1)/2): it adds the block
+    // D_{mu nu} x_nu to its own row and forms the partner row's term
+    // D_{nu mu} x_mu (the same block), which joins a register accumulator that
+    // moves one lane up (mod N) per step, so that after step k lane j holds
+    // terms for row j + k - 1. For even N the pairs at k = N/2 are met from
+    // both ends, so that step adds own-row terms only. FP32 runs of 8 steps
+    // are accumulated in FP64, the moving accumulator after a permute to the
+    // lane of its row. y = x + the sum (identity diagonal blocks).
+    __device__ void apply_pairs(const double v[3], double y[3]) {
+        RpyW<float> W;
+        W.init(P.rpy_f);
+        // one base pointer for the six arrays: positions at +0, +FB, +2FB and
+        // the operand at +wo, +wo + 2 BLOCK, +wo + 4 BLOCK (floats), so each
+        // step needs one address and six immediate offsets
+        constexpr int FB = S::FB;
+        const lds_cptr<float> base = as_lds(&sm.Rf[0][0]);
+        const int wo = (int)(&sm.W[0][0] - &sm.Rf[0][0]);
+        const int me = own ? tid : 0;
+        const lds_cptr<float> mine = base + me;
+        const float rx = mine[0], ry = mine[FB], rz = mine[2 * FB];
+        const float xi = mine[wo], yi = mine[wo + 2 * BLOCK], zi = mine[wo + 4 * BLOCK];
+        const int K = (N - 1) / 2;
+        // lane j takes the moving accumulator from lane j - 1 (mod N)
+        const int rot = own ? 4 * (tid == 0 ? N - 1 : tid - 1) : 4 * tid;
+        auto perm = [](int addr, float x) {
+            return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+        };
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        // own-row term of the pair (mu, mu + k); with `both` also the partner's
+        auto pair = [&](int k, bool both, float& fx, float& fy, float& fz) {
+            const lds_cptr<float> q = mine + k;
+            const float bx = q[0] - rx, by = q[FB] - ry, bz = q[2 * FB] - rz;
+            float om1, om2;
+            W.weights(bx * bx + by * by + bz * bz, om1, om2);
+            const float xx = q[wo], xy = q[wo + 2 * BLOCK], xz = q[wo + 4 * BLOCK];
+            const float t = om2 * (bx * xx + by * xy + bz * xz);
+            fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
+            fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);
+            fz = fmaf(om1, xz, fz); fz = fmaf(t, bz, fz);
+            if (both) {
+                const float u = om2 * (bx * xi + by * yi + bz * zi);
+                sx = fmaf(om1, xi, sx); sx = fmaf(u, bx, sx);
+                sy = fmaf(om1, yi, sy); sy = fmaf(u, by, sy);
+                sz = fmaf(om1, zi, sz); sz = fmaf(u, bz, sz);
+                sx = perm(rot, sx); sy = perm(rot, sy); sz = perm(rot, sz);
+            }
+        };
+        // after step kend the moving terms are for row j + kend: bring them to
+        // that lane and fold both FP32 runs into the FP64 sums
+        auto flush = [&](int kend, float fx, float fy, float fz) {
+            ax += (double)fx; ay += (double)fy; az += (double)fz;
+            int from = tid;
+            if (own && kend > 0) { from = tid - kend; if (from < 0) from += N; }
+            ax += (double)perm(4 * from, sx);
+            ay += (double)perm(4 * from, sy);
+            az += (double)perm(4 * from, sz);
+            sx = sy = sz = 0.0f;
+        };
+        if ((N & 1) == 0 && N >= 2) {
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+            pair(N / 2, false, fx, fy, fz);
+            ax += (double)fx; ay += (double)fy; az += (double)fz;
+        }
+        int k = K;
+        for (; k >= 8; k -= 8) {
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pair(k - u, true, fx, fy, fz);
+            flush(k - 8, fx, fy, fz);
+        }
+        if (k > 0) {
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+            for (; k > 0; --k) pair(k, true, fx, fy, fz);
+            flush(0, fx, fy, fz);
+        }
+        if (own) { y[0] = v[0] + ax; y[1] = v[1] + ay; y[2] = v[2] + az; }
+        else { y[0] = y[1] = y[2] = 0.0; }
     }
 
     // Spring force on the beads from the bonds held by threads tid < N-1
@@ -385,7 +484,8 @@ struct Chain {
         if (et != 1 && et != 2 && et != 3) return;
         if constexpr (MIXED) {
             __syncthreads();
-            for (int d = 0; d < 3; ++d) sm.Xf[d][tid] = own ? (float)(C[d][tid] - C[d][0]) : 0.0f;
+            if (own)
+                for (int d = 0; d < 3; ++d) sm.W[d][tid] = (float)(C[d][tid] - C[d][0]);
             __syncthreads();
         }
         if (!own) return;
@@ -424,7 +524,7 @@ struct Chain {
         const int cd = P.cd;
         lds_cptr<T> x0, x1, x2;
         if constexpr (MIXED) {
-            x0 = as_lds(sm.Xf[0]); x1 = as_lds(sm.Xf[1]); x2 = as_lds(sm.Xf[2]);
+            x0 = as_lds(sm.W[0]); x1 = as_lds(sm.W[1]); x2 = as_lds(sm.W[2]);
         } else {
             x0 = as_lds(C[0]); x1 = as_lds(C[1]); x2 = as_lds(C[2]);
         }
@@ -608,12 +708,25 @@ struct Chain {
             if (tid == 0) sm.Alpha[j] = a;
             k = j + 1;
             for (int d = 0; d < 3; ++d) w[d] -= a * v[d] + b * vprev[d];
-            for (int q = 0; q <= j; ++q) {
-                double uq[3] = {0.0, 0.0, 0.0};
-                if (own)
-                    for (int d = 0; d < 3; ++d) uq[d] = Vb[(size_t)q * n + 3 * tid + d];
-                const double dq = bsum(dot3(w, uq));
-                for (int d = 0; d < 3; ++d) w[d] -= dq * uq[d];
+            // full reorthogonalisation, modified Gram-Schmidt in the
+            // reference's order (q = 0..j, the last being v itself). The
+            // basis vectors do not depend on w, so they are fetched four at a
+            // time ahead of the dependent chain of dot products and updates.
+            for (int q0 = 0; q0 <= j; q0 += 4) {
+                double u[4][3];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int q = q0 + t;
+                    for (int d = 0; d < 3; ++d)
+                        u[t][d] = (q == j) ? v[d]
+                                : (own && q < j) ? Vb[(size_t)q * n + 3 * tid + d] : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (q0 + t > j) break;
+                    const double dq = bsum(dot3(w, u[t]));
+                    for (int d = 0; d < 3; ++d) w[d] -= dq * u[t][d];
+                }
             }
             b = sqrt(bsum(dot3(w, w)));
             if (b < 1.0e-10) break;

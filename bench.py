@@ -120,6 +120,22 @@ class HipEvents:
         return float(ms.value)
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of `nbytes` (the runtime's blit kernel): the
+    known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert h.hipMalloc(byref(src), nbytes) == 0 and h.hipMalloc(byref(dst), nbytes) == 0
+    assert h.hipMemcpy(dst, src, nbytes, 3) == 0          # hipMemcpyDeviceToDevice
+    assert h.hipDeviceSynchronize() == 0
+    h.hipFree(src)
+    h.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+
+
 def run_gpu(lib, precision, steps, warmup, R0, K, ev):
     phys = headline_phys()
     n_traj = R0.shape[0]
@@ -214,7 +230,12 @@ def main():
     ap.add_argument("--traj", type=int, default=N_TRAJ)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
               f"(trajectories would shard by index across GPUs; see DESIGN.md). Got --gpus {args.gpus}.",
