@@ -176,6 +176,12 @@ class GPUUnavailable(RuntimeError):
     """The HIP library is not built, or no MI355X-class GPU is visible."""
 
 
+class CServerStats(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int64), ("running", ctypes.c_int64),
+                ("active_trajectories", ctypes.c_int64), ("bytes_reserved", ctypes.c_int64),
+                ("bytes_in_use", ctypes.c_int64), ("bead_steps_done", ctypes.c_int64)]
+
+
 def _declare_gpu(lib):
     P = POINTER
     lib.bdsim_init.argtypes = [c_int]
@@ -198,6 +204,16 @@ def _declare_gpu(lib):
     lib.bdsim_external_force_batch.argtypes = [P(CPhys), c_int, c_double, _dp]
     lib.bdsim_raw_noise_batch.argtypes = [c_int, c_int, c_int, P(CRng), _dp]
     lib.bdsim_rpy_apply_batch.argtypes = [c_int, c_int, c_double, c_int, _dp, _dp, _dp]
+    lib.bdsim_server_create.argtypes = [P(c_void_p), c_int, ctypes.c_int64, c_int]
+    lib.bdsim_server_submit.argtypes = [c_void_p, ctypes.c_int64, c_int, c_int, _dp, P(CPhys),
+                                        P(CSim), P(CRng)]
+    lib.bdsim_server_poll.argtypes = [c_void_p, c_int]
+    lib.bdsim_server_completed.argtypes = [c_void_p, P(ctypes.c_int64), c_int]
+    lib.bdsim_server_result.argtypes = [c_void_p, ctypes.c_int64, _dp, P(CRng), _ip,
+                                        P(CCounters)]
+    lib.bdsim_server_query.argtypes = [c_void_p, P(CServerStats)]
+    lib.bdsim_server_destroy.argtypes = [c_void_p]
+    lib.bdsim_server_destroy.restype = None
     return lib
 
 

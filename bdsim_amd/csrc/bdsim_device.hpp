@@ -65,6 +65,11 @@ struct StepLog {                    // mirrors bdsim_step_log
 
 struct RngState { int32_t ix, iy, seed; };
 
+// Where a resumable segment stands between launches: the accumulated step
+// time (the reference's `t += dt`), whether the warm Lanczos seed in the
+// workspace is valid, and whether the segment has ended (or failed).
+struct SegState { double t; int32_t seed_valid, done; };
+
 enum : int32_t { ST_OK = 0, ST_NOT_PD = 1, ST_RTSAFE_UNBRACKETED = 2, ST_RTSAFE_MAXIT = 3,
                  ST_EXACT_SQRT = 4, ST_WLCB_SIGMA = 5 };
 

@@ -48,6 +48,10 @@ struct KArgs {
     int32_t sim_override;       // nonzero: use the sim fields below for all
     int32_t com;
     double t0, t1, dt, tol;
+    // resumable segments (null seg: each launch runs whole segments)
+    SegState* seg;              // per trajectory
+    int32_t* done_out;          // per workgroup: 1 once its segment has ended
+    int32_t max_steps;          // steps per launch at most
 };
 
 template <int BLOCK, bool MIXED>
@@ -886,7 +890,8 @@ struct Chain {
             for (int d = 0; d < 3; ++d) R[d] = A.R[P.R_off + 3 * tid + d];
         int32_t ix = A.rng[ti].ix, iy = A.rng[ti].iy, rseed = A.rng[ti].seed;
         bool rng_ready = false;
-        bool seed_valid = false;
+        const bool resume = A.seg != nullptr;
+        bool seed_valid = resume && A.seg[ti].seed_valid != 0;
         int status = ST_OK;
 
         const double t0 = A.sim_override ? A.t0 : P.t0;
@@ -898,8 +903,13 @@ struct Chain {
         StepLog* logp = (A.log && P.log_off >= 0) ? A.log + P.log_off : nullptr;
         StepLog& lg = sm.lg;
         int step = 0;
+        // a resumed segment continues from its stored time, bit for bit the
+        // sequence of one uninterrupted call (the warm seed stays in the
+        // workspace, the generator state in A.rng)
+        double t = resume ? A.seg[ti].t : t0;
+        int budget = resume ? A.max_steps : 0x7fffffff;
 
-        for (double t = t0; t <= t1 + dt / 2.0; t += dt) {
+        for (; t <= t1 + dt / 2.0 && budget > 0; t += dt, --budget) {
             if (tid == 0) { lg = StepLog{}; lg.t = t; }
             // recenter (integrator.cpp:88-93): sequential sums, as the reference
             if (com) {
@@ -1076,6 +1086,11 @@ struct Chain {
             A.rng[ti].seed = rseed;
             A.status[ti] = status;
             A.cnt[ti] = sm.cnt;
+            if (resume) {
+                const int32_t done = (status != ST_OK || !(t <= t1 + dt / 2.0)) ? 1 : 0;
+                A.seg[ti] = SegState{t, seed_valid ? 1 : 0, done};
+                A.done_out[blockIdx.x] = done;
+            }
         }
     }
 
@@ -1103,7 +1118,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)
     __shared__ Smem<BLOCK, MIXED> sm;
     const int ti = a.list[blockIdx.x];
     const TrajParams* __restrict__ P = a.tp + ti;
+    if (a.seg && a.seg[ti].done) {           // nothing left of this segment
+        if (threadIdx.x == 0) a.done_out[blockIdx.x] = 1;
+        return;
+    }
     init_counters(sm);
+    if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];   // counts run on across launches
     Chain<BLOCK, MIXED> c(sm, *P, a);
     c.run(ti);
 }
@@ -1824,3 +1844,5 @@ int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision, c
 }
 
 }  // extern "C"
+
+#include "bdsim_server.hpp"

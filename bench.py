@@ -238,7 +238,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
-              f"(trajectories would shard by index across GPUs; see DESIGN.md). Got --gpus {args.gpus}.",
+              f"(trajectories would shard by index across GPUs). Got --gpus {args.gpus}.",
               file=sys.stderr)
         sys.exit(2)
     lib = A.load_gpu()

@@ -217,6 +217,49 @@ int bdsim_batch_fetch(bdsim_batch* b, double* R, bdsim_rng_state* rng,
 void* bdsim_batch_stream(bdsim_batch* b);
 void bdsim_batch_destroy(bdsim_batch* b);
 
+/* ---- serving: jobs admitted into a resident batch while others run ------
+ *
+ * No counterpart in the reference, whose job API is simulate / run_ensemble
+ * (python/bdsim/ensemble.py:179-218,330-379) over one integrate call per
+ * trajectory segment. A job is n_traj trajectories of one chain length, each
+ * integrating ONE segment [sim->time_start, sim->time_end] from its own R0
+ * and rng: the result is identical, bit for bit, to bdsim_integrate_batch
+ * on the same job, whatever else is resident. Memory for slots, positions,
+ * workspaces and tables is reserved once (hbm_bytes) and handed to queued
+ * jobs in arrival order as their footprint fits (a later job may pass one
+ * that does not fit yet). Trajectories run in launches of at most
+ * steps_per_launch steps (scaled down by BLOCK/64 for longer chains); a job
+ * submitted meanwhile joins at its group's next launch. Groups of different
+ * chain-length buckets run on their own streams, concurrently.
+ * submit may be called from another thread than poll.                       */
+typedef struct bdsim_server bdsim_server;
+
+typedef struct bdsim_server_stats {
+    int64_t queued, running, active_trajectories;
+    int64_t bytes_reserved, bytes_in_use;   /* positions, workspaces, tables */
+    int64_t bead_steps_done;                /* over completed jobs           */
+} bdsim_server_stats;
+
+int bdsim_server_create(bdsim_server** out, int precision, int64_t hbm_bytes,
+                        int steps_per_launch);
+/* Queues a job (inputs are copied). BDSIM_ENOMEM if it could never fit. */
+int bdsim_server_submit(bdsim_server* s, int64_t job_id, int n_traj, int nbeads,
+                        const double* R0, const bdsim_phys* phys,
+                        const bdsim_sim* sim, const bdsim_rng_state* rng);
+/* Services every group whose launch has finished (collect, admit, launch
+ * again), waiting up to wait_us for a job to complete. Returns the number of
+ * completed jobs not yet taken by bdsim_server_completed, or < 0 on error. */
+int bdsim_server_poll(bdsim_server* s, int wait_us);
+/* Takes up to max_ids completed job ids, oldest first; returns the count. */
+int bdsim_server_completed(bdsim_server* s, int64_t* ids, int max_ids);
+/* Results of a completed job (any pointer may be NULL), then forgets it:
+ * R_out [n_traj*N*3], rng_out / status / counters [n_traj].                 */
+int bdsim_server_result(bdsim_server* s, int64_t job_id, double* R_out,
+                        bdsim_rng_state* rng_out, int32_t* status,
+                        bdsim_counters* counters);
+int bdsim_server_query(bdsim_server* s, bdsim_server_stats* stats);
+void bdsim_server_destroy(bdsim_server* s);
+
 /* ---- forces and pieces, batched over chains of one length -------------- */
 
 /* R host [n_traj*N*3] -> F host [n_traj*N*3]. */
