@@ -178,6 +178,52 @@ def run_gpu(lib, precision, steps, warmup, R0, K, ev):
                 corrector_per_step=float(sum(c.corrector_iters for c in cnt) / max(st.sum(), 1)))
 
 
+# ---- HBM traffic ------------------------------------------------------------
+
+COUNTERS_PROFILE = os.path.join(REPO, "profiles", "k_integrate_headline_counters.md")
+
+
+def traffic_per_chain_step(form):
+    """(read, write) HBM bytes per chain-step of k_integrate on this workload,
+    from the copy-calibrated FETCH_SIZE / WRITE_SIZE counter runs recorded in
+    profiles/k_integrate_headline_counters.md, or None where that is absent."""
+    import re
+    try:
+        text = open(COUNTERS_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic-per-chain-step mixed read=(\d+) write=(\d+) fp64 read=(\d+) write=(\d+)",
+                  text)
+    if not m:
+        return None
+    v = [int(x) for x in m.groups()]
+    return (v[0], v[1]) if form == "mixed" else (v[2], v[3])
+
+
+# ---- serving ---------------------------------------------------------------
+
+def serving_leg(n_jobs, step_scale):
+    """The serving stream replayed in real time on the mixed form: Poisson
+    arrivals at 50 jobs/s (seed 3), N log-uniform in [10, 500], 1..64
+    trajectories per job, n_steps log-uniform in [1e3, 1e5] scaled by
+    step_scale, Wi log-uniform in [0.1, 100], h* uniform in [0, 0.3], FENE
+    b=100, Gaussian EV; latency = arrival to result on the host."""
+    from bdsim_amd import serving as S
+    jobs = S.c5_stream(n_jobs, step_scale=step_scale)
+    print(f"bench: serving {n_jobs} jobs", file=sys.stderr, flush=True)
+    with S.Server(A.PRECISION_MIXED, hbm_bytes=32 << 30, steps_per_launch=256) as srv:
+        rep = S.run_stream(srv, jobs)
+    bad = sum(int(np.sum(r.status != 0)) for r in rep["results"].values())
+    return {"p50_s": rep["p50_s"], "p99_s": rep["p99_s"], "makespan_s": rep["makespan_s"],
+            "bead_steps": rep["bead_steps"], "bead_steps_per_s": rep["bead_steps_per_s"],
+            "failed_trajectories": bad,
+            "sample": (f"first {n_jobs} jobs of the serving stream (Poisson 50 jobs/s, seed 3; "
+                       f"N log-uniform [10,500], 1..64 trajectories, Wi log-uniform [0.1,100], "
+                       f"h* uniform [0,0.3], FENE b=100, Gauss EV, dt=1e-3) with n_steps "
+                       f"log-uniform [1e3,1e5] scaled by {step_scale:g}; mixed form, "
+                       f"arrivals replayed in real time, latency from arrival to host result")}
+
+
 # ---- CPU baseline ----------------------------------------------------------
 
 def _cpu_worker(args):
@@ -230,6 +276,9 @@ def main():
     ap.add_argument("--traj", type=int, default=N_TRAJ)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=40)
+    ap.add_argument("--serving-step-scale", type=float, default=0.02)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -258,6 +307,7 @@ def main():
     value = bead_steps / (r["kernel_ms"] / 1e3)
     achieved = r["flops"] / (r["kernel_ms"] / 1e3) / 1e12
     cpu = None if args.no_cpu else cpu_baseline()
+    traffic = traffic_per_chain_step(main_form)
     out = {
         "metric": "bead-steps/s 4096xN=50 RPY-Chebyshev, mixed vs FP64 GPU; p99 job latency",
         "value": value,
@@ -276,7 +326,12 @@ def main():
                    "form": main_form, "n_traj": args.traj, "n_beads": N_BEADS},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[main_form],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_form],
-                     "traffic": None},
+                     "traffic": (None if traffic is None else
+                                 float(sum(traffic)) * args.traj * args.steps),
+                     "traffic_note": ("HBM bytes of the timed launch: per chain-step read + "
+                                      "write from copy-calibrated FETCH_SIZE (x2.0) / "
+                                      "WRITE_SIZE (x1.0) runs, "
+                                      "profiles/k_integrate_headline_counters.md")},
         "cpu_baseline": cpu,
     }
     if "fp64" in res and main_form != "fp64":
@@ -289,7 +344,10 @@ def main():
     out["counters"] = {k: {kk: v[kk] for kk in ("matvecs_per_step", "corrector_per_step",
                                                  "lanczos_cold", "fallbacks", "status_ok",
                                                  "finite", "wall_s")} for k, v in res.items()}
-    out["p99_job_latency_s"] = None
+    srv = None if args.no_serving else serving_leg(args.serving_jobs, args.serving_step_scale)
+    out["p50_job_latency_s"] = srv["p50_s"] if srv else None
+    out["p99_job_latency_s"] = srv["p99_s"] if srv else None
+    out["serving"] = srv
     print(json.dumps(out), flush=True)
 
 
