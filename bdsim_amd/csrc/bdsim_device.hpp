@@ -135,12 +135,20 @@ __device__ __forceinline__ double block_sum(double v, double* sRed, int& par) {
     }
 }
 
+// A pointer into device memory typed as such (global_load / global_store):
+// pointers that reach the kernel inside a struct are otherwise generic, and
+// generic (flat) accesses also wait on the LDS counter.
+template <typename T>
+using gbl_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gbl_ptr<T> as_gbl(T* p) { return (gbl_ptr<T>)p; }
+
 // ---- RNG: Numerical Recipes ran_1 with exact jump-ahead -------------------
 // Thread mu needs draws 3mu+1..3mu+3 of the step. The xorshift is linear over
 // GF(2): state_k = T^k state_0, applied with the 32 columns of T^(3mu). The
 // minstd part is iy_k = iy_0 * 16807^k mod (2^31-1). Both tables are built on
 // the host. The draws then follow rng.cpp:41-70 exactly.
-__device__ __forceinline__ uint32_t gf2_apply(const uint32_t* __restrict__ cols, uint32_t x) {
+__device__ __forceinline__ uint32_t gf2_apply(gbl_ptr<const uint32_t> cols, uint32_t x) {
     uint32_t r = 0;
 #pragma unroll
     for (int i = 0; i < 32; ++i) r ^= cols[i] & (0u - ((x >> i) & 1u));
@@ -179,7 +187,7 @@ __device__ __forceinline__ void rng_maybe_init(int32_t& ix, int32_t& iy, int32_t
 
 // Three Ottinger deviates of bead mu (mobility.cpp:18-29) and the generator
 // state after them. Contraction is off so X is bit-identical to the CPU's.
-__device__ inline void bead_noise(const uint32_t* __restrict__ jx, const uint32_t* __restrict__ jy,
+__device__ inline void bead_noise(gbl_ptr<const uint32_t> jx, gbl_ptr<const uint32_t> jy,
                                   int mu, int32_t ix0, int32_t iy0, double X[3],
                                   int32_t& ix_out, int32_t& iy_out) {
 #pragma clang fp contract(off)
@@ -525,7 +533,11 @@ struct RpyW {
     }
 
     __device__ __forceinline__ void weights(T sq, T& om1, T& om2) const {
-        sq = fmax(sq, clamp);
+        weights_floor(fmax(sq, clamp), om1, om2);
+    }
+
+    // sq already at least the clamp (a separation formed as 1e-12 + |b|^2)
+    __device__ __forceinline__ void weights_floor(T sq, T& om1, T& om2) const {
         const T ir = rsq(sq);
         const T ir2 = ir * ir;
         const T p = K1 * ir;

@@ -98,7 +98,7 @@ __device__ inline int sturm_count(int k, const double* a, const double* b, doubl
 
 // hydrodynamics.cpp:106-136; c, dd, y are this lane's rows of the workspace.
 __device__ inline void tridiag_eigenvector(int m, const double* a, const double* b, double lambda,
-                                           double* c, double* dd, double* y) {
+                                           gbl_ptr<double> c, gbl_ptr<double> dd, gbl_ptr<double> y) {
     const double scale = (fabs(lambda) < 1.0) ? 1.0 : fabs(lambda);
     const double shift = lambda - 1.0e-7 * scale;
     for (int i = 0; i < m; ++i) y[i] = 1.0 / sqrt((double)m);
@@ -172,6 +172,108 @@ __device__ __noinline__ int solve_connector_ool(double g0, double g1, double g2,
     return st;
 }
 
+// Rare-path helpers kept out of line. They take what they use explicitly
+// (no Chain object), so the integrator's state stays in registers around the
+// calls.
+struct D3 { double v[3]; };
+
+// Dense RPY tensor into the workspace (hydrodynamics.cpp:12-54, same entry
+// formula and operand order), then its lower Cholesky factor in place
+// (linalg.cpp:91-107, same per-entry sums, computed column by column).
+// Returns false on a non-positive pivot, with its index/value counted.
+template <int BLOCK, bool MIXED>
+__device__ __noinline__ bool dense_factor_ool(Smem<BLOCK, MIXED>& sm, gbl_ptr<double> Dm, int N,
+                                              double hstar) {
+    const int n = 3 * N, tid = threadIdx.x;
+    const bool own = tid < N;
+    const double RPI = 1.7724538509055160273;
+    const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
+                 C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75;
+    __syncthreads();
+    if (own) {
+        const int mu = tid;
+        for (int nu = 0; nu < N; ++nu) {
+            if (nu == mu) {
+                for (int p = 0; p < 3; ++p)
+                    for (int q = 0; q < 3; ++q) Dm[(size_t)(3 * mu + p) * n + 3 * nu + q] = p == q ? 1.0 : 0.0;
+                continue;
+            }
+            const int lo = mu < nu ? mu : nu, hi_ = mu < nu ? nu : mu;
+            double b[3];
+            for (int d = 0; d < 3; ++d) b[d] = sm.R[d][hi_] - sm.R[d][lo];
+            double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            if (sq < 1.0e-12) sq = 1.0e-12;
+            const double rs = sqrt(sq);
+            const double hsbyrs = hstar / rs, rsbyhs = rs / hstar;
+            const double hsbyrs2 = hsbyrs * hsbyrs;
+            double om1, om2;
+            if (rsbyhs >= TRPI) {
+                om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
+                om2 = RPI3by4 * (hsbyrs / (rs * rs)) * (1.0 - TPI * hsbyrs2);
+            } else {
+                om1 = 1.0 - C3 * rsbyhs;
+                om2 = C7 * rsbyhs / (rs * rs);
+            }
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) {
+                    // entry (3mu+p, 3nu+q); reference index i is the lower bead's
+                    const int i = (mu == lo) ? p : q, j = (mu == lo) ? q : p;
+                    Dm[(size_t)(3 * mu + p) * n + 3 * nu + q] = om2 * b[i] * b[j] + (i == j ? om1 : 0.0);
+                }
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (tid == 0) {
+            double sum = Dm[(size_t)j * n + j];
+            for (int k = 0; k < j; ++k) sum -= Dm[(size_t)j * n + k] * Dm[(size_t)j * n + k];
+            const bool ok = sum > 0.0;
+            sm.ScalI[4] = ok ? 1 : 0;
+            sm.Scal[15] = sum;
+            if (ok) Dm[(size_t)j * n + j] = sqrt(sum);
+        }
+        __syncthreads();
+        if (!sm.ScalI[4]) {
+            if (tid == 0) {
+                sm.cnt.fail_index = j;
+                sm.cnt.fail_value = sm.Scal[15];
+            }
+            return false;
+        }
+        const double Ljj = Dm[(size_t)j * n + j];
+        for (int i = j + 1 + tid; i < n; i += BLOCK) {
+            double sum = Dm[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) sum -= Dm[(size_t)i * n + k] * Dm[(size_t)j * n + k];
+            Dm[(size_t)i * n + j] = sum / Ljj;
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// y = L x with the factor in the workspace (linalg.cpp:109-117), own rows.
+template <int BLOCK, bool MIXED>
+__device__ __noinline__ D3 lower_matvec_ool(Smem<BLOCK, MIXED>& sm, gbl_ptr<double> Dm, int N,
+                                            double x0, double x1, double x2) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    sm.V[0][tid] = x0;
+    sm.V[1][tid] = x1;
+    sm.V[2][tid] = x2;
+    __syncthreads();
+    D3 y{{0.0, 0.0, 0.0}};
+    if (tid >= N) return y;
+    const int n = 3 * N;
+    for (int p = 0; p < 3; ++p) {
+        const int i = 3 * tid + p;
+        const gbl_ptr<double> row = Dm + (size_t)i * n;
+        double s = 0.0;
+        for (int j = 0; j <= i; ++j) s += row[j] * sm.V[j % 3][j / 3];
+        y.v[p] = s;
+    }
+    return y;
+}
+
 template <int BLOCK, bool MIXED>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
@@ -183,13 +285,13 @@ struct Chain {
     const bool own;
     const bool hi;
     const WsLayout wl;
-    double* ws;                 // this trajectory's workspace
+    gbl_ptr<double> ws;         // this trajectory's workspace
     int par = 0;
 
     __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
-        ws = A.ws + P.ws_off;
+        ws = as_gbl(A.ws + P.ws_off);
     }
 
     __device__ __forceinline__ double bsum(double v) { return block_sum<NW>(v, sm.Red, par); }
@@ -228,7 +330,7 @@ struct Chain {
     // over all N partners with its own bead included: the self pair evaluates
     // to om1(1e-12) I and cdiag * v restores the identity block (RpyW), so the
     // pair loop has no lane-dependent select.
-    __device__ void apply(const double v[3], double y[3]) {
+    __device__ __forceinline__ void apply(const double v[3], double y[3]) {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
         __syncthreads();
         if constexpr (MIXED) {
@@ -321,7 +423,7 @@ struct Chain {
     // both ends, so that step adds own-row terms only. FP32 runs of 8 steps
     // are accumulated in FP64, the moving accumulator after a permute to the
     // lane of its row. y = x + the sum (identity diagonal blocks).
-    __device__ void apply_pairs(const double v[3], double y[3]) {
+    __device__ __forceinline__ void apply_pairs(const double v[3], double y[3]) {
         RpyW<float> W;
         W.init(P.rpy_f);
         // one base pointer for the six arrays: positions at +0, +FB, +2FB and
@@ -346,8 +448,11 @@ struct Chain {
         auto pair = [&](int k, bool both, float& fx, float& fy, float& fz) {
             const lds_cptr<float> q = mine + k;
             const float bx = q[0] - rx, by = q[FB] - ry, bz = q[2 * FB] - rz;
+            // the walk never meets the self pair: 1e-12 + |b|^2 stands for the
+            // reference's clamp max(|b|^2, 1e-12) (the same value to FP32
+            // resolution wherever |b|^2 > 1e-5)
             float om1, om2;
-            W.weights(bx * bx + by * by + bz * bz, om1, om2);
+            W.weights_floor(fmaf(bz, bz, fmaf(by, by, fmaf(bx, bx, W.clamp))), om1, om2);
             const float xx = q[wo], xy = q[wo + 2 * BLOCK], xz = q[wo + 4 * BLOCK];
             const float t = om2 * (bx * xx + by * xy + bz * xz);
             fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
@@ -592,92 +697,16 @@ struct Chain {
         for (int d = 0; d < 3; ++d) Fe[d] = (Fev[d] + Fb[d]) + Fx[d];
     }
 
-    // Dense RPY tensor into the workspace (hydrodynamics.cpp:12-54, same entry
-    // formula and operand order), then its lower Cholesky factor in place
-    // (linalg.cpp:91-107, same per-entry sums, computed column by column).
-    // Returns false on a non-positive pivot, with its index/value counted.
-    __device__ __noinline__ bool dense_factor() {
-        const int n = n3;
-        double* Dm = ws + wl.dense;
-        const double RPI = 1.7724538509055160273;
-        const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
-                     C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75, hstar = P.hstar;
-        __syncthreads();
-        if (own) {
-            const int mu = tid;
-            for (int nu = 0; nu < N; ++nu) {
-                if (nu == mu) {
-                    for (int p = 0; p < 3; ++p)
-                        for (int q = 0; q < 3; ++q) Dm[(size_t)(3 * mu + p) * n + 3 * nu + q] = p == q ? 1.0 : 0.0;
-                    continue;
-                }
-                const int lo = mu < nu ? mu : nu, hi_ = mu < nu ? nu : mu;
-                double b[3];
-                for (int d = 0; d < 3; ++d) b[d] = sm.R[d][hi_] - sm.R[d][lo];
-                double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-                if (sq < 1.0e-12) sq = 1.0e-12;
-                const double rs = sqrt(sq);
-                const double hsbyrs = hstar / rs, rsbyhs = rs / hstar;
-                const double hsbyrs2 = hsbyrs * hsbyrs;
-                double om1, om2;
-                if (rsbyhs >= TRPI) {
-                    om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
-                    om2 = RPI3by4 * (hsbyrs / (rs * rs)) * (1.0 - TPI * hsbyrs2);
-                } else {
-                    om1 = 1.0 - C3 * rsbyhs;
-                    om2 = C7 * rsbyhs / (rs * rs);
-                }
-                for (int p = 0; p < 3; ++p)
-                    for (int q = 0; q < 3; ++q) {
-                        // entry (3mu+p, 3nu+q); reference index i is the lower bead's
-                        const int i = (mu == lo) ? p : q, j = (mu == lo) ? q : p;
-                        Dm[(size_t)(3 * mu + p) * n + 3 * nu + q] = om2 * b[i] * b[j] + (i == j ? om1 : 0.0);
-                    }
-            }
-        }
-        __syncthreads();
-        for (int j = 0; j < n; ++j) {
-            if (tid == 0) {
-                double sum = Dm[(size_t)j * n + j];
-                for (int k = 0; k < j; ++k) sum -= Dm[(size_t)j * n + k] * Dm[(size_t)j * n + k];
-                const bool ok = sum > 0.0;
-                sm.ScalI[4] = ok ? 1 : 0;
-                sm.Scal[15] = sum;
-                if (ok) Dm[(size_t)j * n + j] = sqrt(sum);
-            }
-            __syncthreads();
-            if (!sm.ScalI[4]) {
-                put(sm.cnt.fail_index, j);
-                put(sm.cnt.fail_value, sm.Scal[15]);
-                return false;
-            }
-            const double Ljj = Dm[(size_t)j * n + j];
-            for (int i = j + 1 + tid; i < n; i += BLOCK) {
-                double sum = Dm[(size_t)i * n + j];
-                for (int k = 0; k < j; ++k) sum -= Dm[(size_t)i * n + k] * Dm[(size_t)j * n + k];
-                Dm[(size_t)i * n + j] = sum / Ljj;
-            }
-            __syncthreads();
-        }
-        return true;
+    // Dense RPY tensor and its lower Cholesky factor in the workspace
+    // (dense_factor_ool); false on a non-positive pivot.
+    __device__ __forceinline__ bool dense_factor() {
+        return dense_factor_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, P.hstar);
     }
 
-    // y = L x with the factor in the workspace (linalg.cpp:109-117), own rows.
-    __device__ __noinline__ void lower_matvec(const double x[3], double y[3]) {
-        __syncthreads();
-        for (int d = 0; d < 3; ++d) sm.V[d][tid] = x[d];
-        __syncthreads();
-        y[0] = y[1] = y[2] = 0.0;
-        if (!own) return;
-        const int n = n3;
-        const double* Dm = ws + wl.dense;
-        for (int p = 0; p < 3; ++p) {
-            const int i = 3 * tid + p;
-            const double* row = Dm + (size_t)i * n;
-            double s = 0.0;
-            for (int j = 0; j <= i; ++j) s += row[j] * sm.V[j % 3][j / 3];
-            y[p] = s;
-        }
+    // y = L x with the factor in the workspace (lower_matvec_ool), own rows.
+    __device__ __forceinline__ void lower_matvec(const double x[3], double y[3]) {
+        const D3 r = lower_matvec_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, x[0], x[1], x[2]);
+        y[0] = r.v[0]; y[1] = r.v[1]; y[2] = r.v[2];
     }
 
     // default Lanczos seed sin(0.7654321 (i+1)) + 0.5 (hydrodynamics.cpp:83-87)
@@ -691,7 +720,7 @@ struct Chain {
     // vmin + vmax (the next seed is built from their sum only).
     __device__ int lanczos(int m, bool warm, double& rmin, double& rmax, double vsum[3]) {
         if (m > n3) m = n3;
-        double* Vb = ws + wl.V;
+        const gbl_ptr<double> Vb = ws + wl.V;
         double v[3], vprev[3] = {0.0, 0.0, 0.0}, w[3];
         if (warm) load(SLOT_SEED, v); else default_seed(v);
         double nv = sqrt(bsum(dot3(v, v)));
@@ -746,14 +775,14 @@ struct Chain {
             }
             return k;
         }
-        double* tri = ws + wl.tri;
+        const gbl_ptr<double> tri = ws + wl.tri;
         if (tid < 64) ritz_multisect(k, sm.Alpha, sm.Beta, sm.Scal);
         __syncthreads();
         rmin = sm.Scal[0];
         rmax = sm.Scal[1];
         // Ritz vectors of the two extremes by inverse iteration (lanes 0, 1)
         if (tid < 2) {
-            double* rows = tri + tid * 3 * LANCZOS_COLD;
+            const gbl_ptr<double> rows = tri + tid * 3 * LANCZOS_COLD;
             tridiag_eigenvector(k, sm.Alpha, sm.Beta, tid == 0 ? rmin : rmax, rows,
                                 rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
         }
@@ -761,8 +790,8 @@ struct Chain {
         // lift the tridiagonal eigenvectors: v = V^T y (hydrodynamics.cpp:264-273)
         double vmin[3] = {0.0, 0.0, 0.0}, vmax[3] = {0.0, 0.0, 0.0};
         if (own) {
-            const double* ymn = tri + 2 * LANCZOS_COLD;
-            const double* ymx = tri + 3 * LANCZOS_COLD + 2 * LANCZOS_COLD;
+            const gbl_ptr<double> ymn = tri + 2 * LANCZOS_COLD;
+            const gbl_ptr<double> ymx = tri + 3 * LANCZOS_COLD + 2 * LANCZOS_COLD;
             for (int j = 0; j < k; ++j) {
                 const double a0 = ymn[j], a1 = ymx[j];
                 for (int d = 0; d < 3; ++d) {
@@ -779,8 +808,8 @@ struct Chain {
     // Chebyshev coefficients of sqrt on the interval (hydrodynamics.cpp:267-279)
     // into the workspace: one coefficient per thread.
     __device__ void cheb_coefficients(int L, double da, double db) {
-        double* coef = ws + wl.coef;
-        double* aux = ws + wl.aux;
+        const gbl_ptr<double> coef = ws + wl.coef;
+        const gbl_ptr<double> aux = ws + wl.aux;
         __syncthreads();
         for (int k = tid; k <= L; k += BLOCK) aux[k] = sqrt(cos(PI * (k + 0.5) / (L + 1)) / da - db / da);
         __syncthreads();
@@ -840,7 +869,7 @@ struct Chain {
         if (L < 2) L = 2;
         const double da = 2.0 / (lmax - lmin);
         const double db = -(lmax + lmin) / (lmax - lmin);
-        const double* coef = ws + wl.coef;
+        const gbl_ptr<double> coef = ws + wl.coef;
         double fd = 0.0;
         for (;;) {
             cheb_coefficients(L, da, db);
@@ -949,7 +978,7 @@ struct Chain {
             {
                 double x0[3] = {0.0, 0.0, 0.0};
                 int32_t nix = ix, niy = iy;
-                if (own) bead_noise(A.jx, A.jy, tid, ix, iy, x0, nix, niy);
+                if (own) bead_noise(as_gbl(A.jx), as_gbl(A.jy), tid, ix, iy, x0, nix, niy);
                 if (tid == N - 1) { sm.ScalI[0] = nix; sm.ScalI[1] = niy; }
                 __syncthreads();
                 ix = sm.ScalI[0];
@@ -1173,7 +1202,7 @@ __global__ void __launch_bounds__(BLOCK) k_noise(KArgs a, int N, int n_steps, do
         double X[3];
         int32_t nix = ix, niy = iy;
         if (tid < N) {
-            bead_noise(a.jx, a.jy, tid, ix, iy, X, nix, niy);
+            bead_noise(as_gbl(a.jx), as_gbl(a.jy), tid, ix, iy, X, nix, niy);
             for (int d = 0; d < 3; ++d)
                 out[((int64_t)ti * n_steps + st) * 3 * N + 3 * tid + d] = X[d];
         }
