@@ -1278,6 +1278,8 @@ struct Ctx {
     bool ready = false;
     int device = -1;
     hipStream_t stream = nullptr;
+    hipStream_t bstream[5] = {};        // one per block-size bucket
+    hipEvent_t bevent[5] = {}, uploaded = nullptr;
     uint32_t* jx = nullptr;
     uint32_t* jy = nullptr;
     DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log;
@@ -1310,6 +1312,11 @@ int ensure_init() {
     if (g.device < 0) g.device = 0;
     HIP_TRY(hipSetDevice(g.device));
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    for (int b = 0; b < 5; ++b) {
+        HIP_TRY(hipStreamCreateWithFlags(&g.bstream[b], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&g.bevent[b], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming));
     std::vector<uint32_t> jx, jy;
     build_jump_tables(jx, jy);
     HIP_TRY(hipMalloc(&g.jx, jx.size() * sizeof(uint32_t)));
@@ -1444,9 +1451,9 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a) 
 }
 
 template <bool MIXED>
-int launch_bucket(int block, int count, const KArgs& a) {
+int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a) {
     if (count == 0) return 0;
-    launch_integrate<MIXED>(block, dim3(count), g.stream, a);
+    launch_integrate<MIXED>(block, dim3(count), stream, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1513,13 +1520,19 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
     }
     if (blob.empty()) blob.push_back(0.0);
     if (ext.empty()) ext.push_back(ExtEntry{});
-    // buckets by block size
+    // Buckets by block size, each launched on its own stream so that buckets
+    // overlap on the device; within a bucket the longest chains go first
+    // (workgroups are dispatched in order, so the short ones fill in behind).
     std::vector<int32_t> order;
     std::vector<int> counts(5, 0);
     const int blocks[5] = {64, 128, 256, 512, 1024};
-    for (int b = 0; b < 5; ++b)
+    for (int b = 0; b < 5; ++b) {
+        const size_t first = order.size();
         for (int i = 0; i < n_traj; ++i)
             if (block_for(nbeads[i]) == blocks[b]) { order.push_back(i); ++counts[b]; }
+        std::stable_sort(order.begin() + first, order.end(),
+                         [&](int32_t x, int32_t y) { return nbeads[x] > nbeads[y]; });
+    }
 
     if (int rc = g.tp.ensure(sizeof(TrajParams) * n_traj)) return rc;
     if (int rc = g.list.ensure(sizeof(int32_t) * n_traj)) return rc;
@@ -1557,13 +1570,22 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
     a.jx = g.jx;
     a.jy = g.jy;
     a.sim_override = 0;
-    int start = 0;
-    for (int b = 0; b < 5; ++b) {
-        a.list = as<int32_t>(g.list) + start;
-        int rc = precision == BDSIM_PRECISION_MIXED ? launch_bucket<true>(blocks[b], counts[b], a)
-                                                    : launch_bucket<false>(blocks[b], counts[b], a);
-        if (rc) return rc;
-        start += counts[b];
+    int starts[5];
+    for (int b = 0, st = 0; b < 5; ++b) { starts[b] = st; st += counts[b]; }
+    HIP_TRY(hipEventRecord(g.uploaded, g.stream));
+    for (int b = 4; b >= 0; --b) {                // largest blocks first
+        if (counts[b] == 0) continue;
+        HIP_TRY(hipStreamWaitEvent(g.bstream[b], g.uploaded, 0));
+        a.list = as<int32_t>(g.list) + starts[b];
+        int rc = precision == BDSIM_PRECISION_MIXED
+                     ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a)
+                     : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a);
+        if (rc) {
+            (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
+            return rc;
+        }
+        HIP_TRY(hipEventRecord(g.bevent[b], g.bstream[b]));
+        HIP_TRY(hipStreamWaitEvent(g.stream, g.bevent[b], 0));
     }
     HIP_TRY(hipMemcpyAsync(R, g.R.p, sizeof(double) * roff, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipMemcpyAsync(rng, g.rng.p, sizeof(RngState) * n_traj, hipMemcpyDeviceToHost, g.stream));

@@ -42,55 +42,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 from bdsim_amd import _abi as A  # noqa: E402
+from bdsim_amd.workloads import (N_BEADS, N_TRAJ, WI, headline_phys, kappas, shear_rate,  # noqa: E402
+                                 long_workload, start_configs, uneven_workload)
 
-N_BEADS = 50
-N_TRAJ = 4096
-WI = [0.1, 1.0, 10.0, 100.0]
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}
-
-
-def headline_phys():
-    c = A.CPhys()
-    c.spring_type = 2
-    c.sqrtb = 10.0
-    c.natural_length = 0.0
-    c.ev_type = 3
-    c.zstar = 0.0
-    c.dstar = 1.0
-    c.min_cutoff = 0.7
-    c.max_cutoff = 1.5
-    c.contour_dist_for_ev = 1
-    c.bend_type = 0
-    c.flow.constant = 1
-    c.number_of_beads = N_BEADS
-    c.hstar = 0.3
-    c.hi_method = 0
-    c.ncheb_multiplier = 1.0
-    c.fd_err_max = 0.0025
-    c.equilibration = 0
-    return c
-
-
-def start_configs(n_traj, N):
-    R = np.empty((n_traj, N, 3))
-    for i in range(n_traj):
-        rs = np.random.default_rng(i)
-        b = rs.normal(scale=1.0, size=(N - 1, 3))
-        R[i, 0] = 0.0
-        R[i, 1:] = np.cumsum(b, axis=0)
-        R[i] -= R[i].mean(axis=0)
-    return np.ascontiguousarray(R)
-
-
-def shear_rate(wi, N=N_BEADS):
-    return wi * np.sin(np.pi / (2.0 * N)) ** 2
-
-
-def kappas(n_traj):
-    K = np.zeros((n_traj, 9))
-    for i in range(n_traj):
-        K[i, 1] = shear_rate(WI[i % len(WI)])
-    return np.ascontiguousarray(K)
 
 
 class HipEvents:
@@ -176,6 +131,34 @@ def run_gpu(lib, precision, steps, warmup, R0, K, ev):
                 lanczos_cold=int(sum(c.lanczos_cold for c in cnt)),
                 fallbacks=int(sum(c.fallbacks for c in cnt)),
                 corrector_per_step=float(sum(c.corrector_iters for c in cnt) / max(st.sum(), 1)))
+
+
+def run_workload(lib, name, precision, steps, warmup):
+    """Integrate a mixed-size workload through bdsim_integrate_batch: one warm-up
+    segment, then one timed segment; the time includes the host <-> device
+    copies of positions and states (a few MB)."""
+    N, R, phys, dt = uneven_workload() if name == "uneven" else long_workload()
+    n = len(N)
+    Rf = np.ascontiguousarray(np.concatenate([r.ravel() for r in R]))
+    ph = (A.CPhys * n)(*phys)
+    rng = (A.CRng * n)(*[A.CRng(1, -1, i) for i in range(n)])
+    st = np.zeros(n, dtype=np.int32)
+    cnt = (A.CCounters * n)()
+
+    def segment(t0, k):
+        sims = (A.CSim * n)(*[A.CSim(t0, t0 + (k - 1) * dt, dt, 1e-3, 1)] * n)
+        t = time.perf_counter()
+        A.check(lib, lib.bdsim_integrate_batch(n, A.iptr(N), A.dptr(Rf), ph, sims, rng, precision,
+                                               A.iptr(st), cnt, None, 0))
+        return time.perf_counter() - t
+
+    if warmup > 0:
+        segment(0.0, warmup)
+    wall = segment(warmup * dt, steps)
+    mv = sum(int(c.matvecs) * 18.0 * int(N[i]) ** 2 for i, c in enumerate(cnt))
+    return dict(bead_steps_per_s=float(N.sum()) * steps / wall, wall_s=wall, n_traj=n,
+                beads=int(N.sum()), status_ok=int(np.sum(st == 0)), tflops=mv / wall / 1e12,
+                finite=bool(np.all(np.isfinite(Rf))))
 
 
 # ---- HBM traffic ------------------------------------------------------------
@@ -281,9 +264,24 @@ def main():
     ap.add_argument("--serving-step-scale", type=float, default=0.02)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--workload", choices=["headline", "uneven", "long"], default="headline",
+                    help="uneven: 4096 chains with N log-uniform in [10,256]; "
+                         "long: 64 chains of N=1000 (each prints its own line)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload != "headline":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        lib = A.load_gpu()
+        out = {"workload": args.workload, "steps": args.steps, "warmup": args.warmup}
+        for form in args.forms.split(","):
+            prec = A.PRECISION_MIXED if form == "mixed" else A.PRECISION_FP64
+            print(f"bench: {args.workload} {form}", file=sys.stderr, flush=True)
+            out[form] = run_workload(lib, args.workload, prec, args.steps, args.warmup)
+        print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "

@@ -15,6 +15,7 @@ import pytest
 
 import helpers as H
 from bdsim_amd import _abi as A
+from bdsim_amd import workloads as W
 from golden import cases as C
 
 pytestmark = pytest.mark.gpu
@@ -197,6 +198,54 @@ def test_uneven_batch_equals_single_runs(gpu):
         alone, ra, sa, _, _ = H.gpu_batch(gpu, [it])
         assert np.array_equal(alone[0], together[j]) and sa[0] == stt[j]
         assert (ra[0].ix, ra[0].iy) == (rt[j].ix, rt[j].iy)
+
+
+def test_uneven_workload_sample_matches_oracle(gpu, orc):
+    """The uneven workload (N log-uniform in [10, 256], every block size from
+    64 to 256 in one call, buckets launched side by side): 32 of its chains,
+    10 steps, each within 1e-10 of the FP64 oracle with the same RNG state."""
+    N, R, phys, dt = W.uneven_workload(n_traj=32)
+    assert len(set((int(n) + 63) // 64 for n in N)) == 4
+    sim = A.CSim(0.0, 9 * dt, dt, 1e-3, 1)
+    items = [(int(N[i]), R[i], phys[i], sim, A.CRng(1, -1, i)) for i in range(len(N))]
+    out, rngs, status, cnts, _ = H.gpu_batch(gpu, items)
+    assert np.all(status == 0)
+    for i in range(len(N)):
+        st = A.CRng(1, -1, i)
+        want, rc, cnto, _ = H.oracle_integrate(orc, int(N[i]), R[i], phys[i], sim, st)
+        assert rc == 0 and cnts[i].steps == cnto.steps == 10
+        assert rel(out[i], want) <= 1e-10, (i, int(N[i]), rel(out[i], want))
+        assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st.ix, st.iy, st.seed)
+
+
+def test_long_chains_n1000(gpu, orc):
+    """Two chains of the N=1000 workload (FENE, bending, Lennard-Jones, RPY
+    h*=0.2 Chebyshev, 16 waves per chain): 6 steps within 1e-10 of the FP64
+    oracle with the same per-step decisions (Lanczos, Chebyshev degree,
+    corrector passes), and within 1e-5 in the mixed form."""
+    N, R, phys, dt = W.long_workload(n_traj=2)
+    n = int(N[0])
+    sim = A.CSim(0.0, 5 * dt, dt, 1e-3, 1)
+    want, logs_o = [], []
+    for i in range(2):
+        Ro, rc, cnto, lo = H.oracle_integrate(orc, n, R[i], phys[i], sim, A.CRng(1, -1, i), log_cap=8)
+        assert rc == 0 and cnto.steps == 6
+        want.append(Ro)
+        logs_o.append(lo)
+    items = [(n, R[i], phys[i], sim, A.CRng(1, -1, i)) for i in range(2)]
+    out, _, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=8)
+    assert np.all(status == 0)
+    for i in range(2):
+        assert rel(out[i], want[i]) <= 1e-10, (i, rel(out[i], want[i]))
+        for k in range(6):
+            a, b = logs[i * 8 + k], logs_o[i][k]
+            assert (a.lanczos_cold, a.lanczos_steps, a.cheb_L, a.cheb_attempts, a.fallback,
+                    a.corrector_iters) == (b.lanczos_cold, b.lanczos_steps, b.cheb_L,
+                                           b.cheb_attempts, b.fallback, b.corrector_iters), (i, k)
+    out, _, status, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    assert np.all(status == 0)
+    for i in range(2):
+        assert rel(out[i], want[i]) < 1e-5, (i, rel(out[i], want[i]))
 
 
 def test_mixed_precision_within_1e5(gpu, orc):
