@@ -182,6 +182,18 @@ class CServerStats(ctypes.Structure):
                 ("bytes_in_use", ctypes.c_int64), ("bead_steps_done", ctypes.c_int64)]
 
 
+class CObservables(ctypes.Structure):
+    """bdsim_observables: one chain's snapshot observables."""
+    _fields_ = [("rg2", c_double), ("q2", c_double), ("gyration", c_double * 9),
+                ("stretch", c_double * 3), ("stress", c_double * 9), ("com", c_double * 3)]
+
+
+FORCE_TOTAL = 0
+FORCE_SPRING = 1
+FORCE_BENDING = 2
+FORCE_EV = 3
+
+
 def _declare_gpu(lib):
     P = POINTER
     lib.bdsim_init.argtypes = [c_int]
@@ -214,6 +226,8 @@ def _declare_gpu(lib):
     lib.bdsim_server_query.argtypes = [c_void_p, P(CServerStats)]
     lib.bdsim_server_destroy.argtypes = [c_void_p]
     lib.bdsim_server_destroy.restype = None
+    lib.bdsim_observables_batch.argtypes = [c_int, c_int, _dp, P(CPhys), c_int, P(CObservables)]
+    lib.bdsim_batch_observe.argtypes = [c_void_p, c_int, P(CObservables)]
     return lib
 
 

@@ -1234,6 +1234,101 @@ __global__ void __launch_bounds__(BLOCK) k_rpy_apply(KArgs a, const double* x_in
         for (int d = 0; d < 3; ++d) out[off + 3 * tid + d] = y[d];
 }
 
+// ---- snapshot observables (python/bdsim/properties.py:5-52) -----------------
+// One record per chain, the layout of bdsim_observables.
+struct Obs {
+    double rg2, q2;
+    double gyration[9];
+    double stretch[3];
+    double stress[9];
+    double com[3];
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide maximum; shares block_sum's alternating halves of sRed.
+template <int NW>
+__device__ __forceinline__ double block_max(double v, double* sRed, int& par) {
+    v = wave_max(v);
+    if constexpr (NW == 1) {
+        return v;
+    } else {
+        double* buf = sRed + par * NW;
+        par ^= 1;
+        if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
+        __syncthreads();
+        double t = buf[0];
+#pragma unroll
+        for (int i = 1; i < NW; ++i) t = fmax(t, buf[i]);
+        return t;
+    }
+}
+
+// Centre of mass, gyration tensor and its trace, squared end-to-end distance,
+// extent of the centred chain, and the Kramers-Kirkwood stress
+// tau_ij = sum_mu (R_mu - R_c)_i F_mu,j with F the total intramolecular force
+// (spring + EV + bending, bindings/module.cpp:67-76) or one of its parts.
+// per_chain: each chain has its own TrajParams (positions at its R_off);
+// otherwise all chains share a.tp[0] and sit back to back.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_observe(KArgs a, int per_chain, int which, Obs* out) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ Smem<BLOCK, false> sm;
+    const int ti = blockIdx.x;
+    const TrajParams* __restrict__ P = per_chain ? a.tp + ti : a.tp;
+    init_counters(sm);
+    Chain<BLOCK, false> c(sm, *P, a);
+    const int N = P->N, tid = threadIdx.x;
+    const bool own = tid < N;
+    const int64_t off = per_chain ? P->R_off : (int64_t)ti * 3 * N;
+    double R[3] = {0.0, 0.0, 0.0};
+    if (own)
+        for (int d = 0; d < 3; ++d) R[d] = a.R[off + 3 * tid + d];
+    for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
+    __syncthreads();
+    double bond[3] = {0.0, 0.0, 0.0};
+    if (tid < N - 1)
+        for (int d = 0; d < 3; ++d) bond[d] = sm.R[d][tid + 1] - R[d];
+    double Fc[3], Fs[3], Fev[3], Fb[3], F[3];
+    c.spring_bead_force(bond, Fc, Fs);
+    c.ev_force(sm.R, Fev);
+    c.bending_force(sm.R, Fb);
+    for (int d = 0; d < 3; ++d) {
+        if (which == FORCE_SPRING) F[d] = Fs[d];
+        else if (which == FORCE_BENDING) F[d] = Fb[d];
+        else if (which == FORCE_EV) F[d] = Fev[d];
+        else F[d] = Fs[d] + (Fev[d] + Fb[d]);
+        if (!own) F[d] = 0.0;
+    }
+    int par = 0;
+    Obs o;
+    for (int d = 0; d < 3; ++d) o.com[d] = block_sum<NW>(R[d], sm.Red, par) / N;
+    double rc[3];
+    for (int d = 0; d < 3; ++d) rc[d] = own ? R[d] - o.com[d] : 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) {
+            const double gij = block_sum<NW>(rc[i] * rc[j], sm.Red, par) / N;
+            o.gyration[3 * i + j] = gij;
+            o.gyration[3 * j + i] = gij;
+        }
+    o.rg2 = o.gyration[0] + o.gyration[4] + o.gyration[8];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.stress[3 * i + j] = block_sum<NW>(rc[i] * F[j], sm.Red, par);
+    for (int d = 0; d < 3; ++d) {
+        const double hi = block_max<NW>(own ? rc[d] : -INFINITY, sm.Red, par);
+        const double lo = -block_max<NW>(own ? -rc[d] : -INFINITY, sm.Red, par);
+        o.stretch[d] = hi - lo;
+    }
+    double e[3];
+    for (int d = 0; d < 3; ++d) e[d] = sm.R[d][N - 1] - sm.R[d][0];
+    o.q2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    if (tid == 0) out[ti] = o;
+}
+
 }  // namespace bdgpu
 
 
@@ -1282,7 +1377,7 @@ struct Ctx {
     hipEvent_t bevent[5] = {}, uploaded = nullptr;
     uint32_t* jx = nullptr;
     uint32_t* jy = nullptr;
-    DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log;
+    DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log, obs;
 } g;
 
 // Columns of T^(3 mu) for the xorshift (13,17,5) map and 16807^(3 mu) mod M.
@@ -1475,6 +1570,7 @@ struct bdsim_batch {
     RngState* rng = nullptr;
     int32_t* status = nullptr;
     Counters* cnt = nullptr;
+    void* obs = nullptr;            // observables records, allocated on first use
 };
 
 extern "C" {
@@ -1713,18 +1809,19 @@ void bdsim_batch_destroy(bdsim_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     (void)hipFree(b->tp); (void)hipFree(b->list); (void)hipFree(b->R); (void)hipFree(b->ws);
     (void)hipFree(b->blob); (void)hipFree(b->ext); (void)hipFree(b->rng);
-    (void)hipFree(b->status); (void)hipFree(b->cnt);
+    (void)hipFree(b->status); (void)hipFree(b->cnt); (void)hipFree(b->obs);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
 // ---- piecewise entry points ------------------------------------------------
 
-static int forces_common(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
-                         double* F, int which) {
-    std::lock_guard<std::mutex> lk(g_mu);
+// Uploads n_traj chains of one length sharing one PhysParams (positions back
+// to back) and fills the kernel arguments for them. Called with g_mu held.
+static int upload_shared(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
+                         KArgs& a) {
     if (int rc = ensure_init()) return rc;
-    if (n_traj <= 0 || nbeads < 2 || nbeads > MAX_BEADS || !R || !phys || !F)
+    if (n_traj <= 0 || nbeads < 2 || nbeads > MAX_BEADS || !R || !phys)
         return fail(BDSIM_EINVAL, "bad argument");
     std::vector<double> blob;
     std::vector<ExtEntry> ext;
@@ -1743,12 +1840,22 @@ static int forces_common(int n_traj, int nbeads, const double* R, const bdsim_ph
     HIP_TRY(hipMemcpyAsync(g.R.p, R, sizeof(double) * nd, hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipMemcpyAsync(g.blob.p, blob.data(), sizeof(double) * blob.size(), hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipMemcpyAsync(g.ext.p, ext.data(), sizeof(ExtEntry) * ext.size(), hipMemcpyHostToDevice, g.stream));
-    KArgs a{};
+    a = KArgs{};
     a.tp = as<TrajParams>(g.tp);
     a.R = as<double>(g.R);
     a.ws = as<double>(g.ws);
     a.blob = as<double>(g.blob);
     a.ext = as<ExtEntry>(g.ext);
+    return 0;
+}
+
+static int forces_common(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
+                         double* F, int which) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!F) return fail(BDSIM_EINVAL, "bad argument");
+    KArgs a;
+    if (int rc = upload_shared(n_traj, nbeads, R, phys, a)) return rc;
+    const size_t nd = (size_t)n_traj * 3 * nbeads;
     dim3 grid(n_traj);
     switch (block_for(nbeads)) {
         case 64: hipLaunchKernelGGL(k_forces<64>, grid, dim3(64), 0, g.stream, a, which, a.ws); break;
@@ -1760,6 +1867,57 @@ static int forces_common(int n_traj, int nbeads, const double* R, const bdsim_ph
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(F, g.ws.p, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+static int launch_observe(int block, int n_traj, hipStream_t stream, const KArgs& a, int per_chain,
+                          int which, Obs* out) {
+    const dim3 grid(n_traj);
+    switch (block) {
+        case 64: hipLaunchKernelGGL(k_observe<64>, grid, dim3(64), 0, stream, a, per_chain, which, out); break;
+        case 128: hipLaunchKernelGGL(k_observe<128>, grid, dim3(128), 0, stream, a, per_chain, which, out); break;
+        case 256: hipLaunchKernelGGL(k_observe<256>, grid, dim3(256), 0, stream, a, per_chain, which, out); break;
+        case 512: hipLaunchKernelGGL(k_observe<512>, grid, dim3(512), 0, stream, a, per_chain, which, out); break;
+        default: hipLaunchKernelGGL(k_observe<1024>, grid, dim3(1024), 0, stream, a, per_chain, which, out); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static_assert(sizeof(Obs) == sizeof(bdsim_observables), "observables record layout");
+
+int bdsim_observables_batch(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
+                            int force_kind, bdsim_observables* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!out || force_kind < BDSIM_FORCE_TOTAL || force_kind > BDSIM_FORCE_EV)
+        return fail(BDSIM_EINVAL, "bad argument");
+    KArgs a;
+    if (int rc = upload_shared(n_traj, nbeads, R, phys, a)) return rc;
+    if (int rc = g.obs.ensure(sizeof(Obs) * n_traj)) return rc;
+    if (int rc = launch_observe(block_for(nbeads), n_traj, g.stream, a, 0, force_kind, as<Obs>(g.obs)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, g.obs.p, sizeof(Obs) * n_traj, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int bdsim_batch_observe(bdsim_batch* b, int force_kind, bdsim_observables* out) {
+    if (!b || !out || force_kind < BDSIM_FORCE_TOTAL || force_kind > BDSIM_FORCE_EV)
+        return fail(BDSIM_EINVAL, "bad argument");
+    if (!b->obs && hipMalloc(&b->obs, sizeof(Obs) * b->n_traj) != hipSuccess) {
+        b->obs = nullptr;
+        return fail(BDSIM_ENOMEM, "hipMalloc failed for observables");
+    }
+    KArgs a{};
+    a.tp = b->tp;
+    a.R = b->R;
+    a.ws = b->ws;
+    a.blob = b->blob;
+    a.ext = b->ext;
+    if (int rc = launch_observe(b->block, b->n_traj, b->stream, a, 1, force_kind, static_cast<Obs*>(b->obs)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(out, b->obs, sizeof(Obs) * b->n_traj, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
     return 0;
 }
 

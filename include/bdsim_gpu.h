@@ -282,6 +282,37 @@ int bdsim_raw_noise_batch(int n_traj, int nbeads, int n_steps,
 int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision,
                           const double* R, const double* x, double* y);
 
+/* ---- snapshot observables ----------------------------------------------
+ * One record per chain, evaluated on the device where the positions are:
+ * what python/bdsim/properties.py:5-52 computes from a snapshot
+ * (end_to_end_sq, gyration_tensor, radius_of_gyration_sq, stretch,
+ * kramers_stress) together with the force _bdsim.total_force returns
+ * (bindings/module.cpp:67-76) or one of its parts, so that ensemble averages
+ * need no copy of the positions to the host (the snapshots of
+ * python/bdsim/ensemble.py:67-83 reduced to what rheology and the
+ * equilibrium checks read).                                                 */
+enum { BDSIM_FORCE_TOTAL = 0,   /* spring + EV + bending                     */
+       BDSIM_FORCE_SPRING = 1,
+       BDSIM_FORCE_BENDING = 2,
+       BDSIM_FORCE_EV = 3 };
+
+typedef struct bdsim_observables {
+    double rg2;            /* trace of the gyration tensor                   */
+    double q2;             /* |R_N - R_1|^2                                  */
+    double gyration[9];    /* (1/N) sum (R-Rc)(R-Rc)^T, row-major            */
+    double stretch[3];     /* max - min of the centred chain per axis        */
+    double stress[9];      /* tau_ij = sum_mu (R_mu - R_c)_i F_mu,j          */
+    double com[3];         /* centre of mass R_c                             */
+} bdsim_observables;
+
+/* R host [n_traj*N*3] (one PhysParams for all chains) -> out [n_traj]. */
+int bdsim_observables_batch(int n_traj, int nbeads, const double* R,
+                            const bdsim_phys* phys, int force_kind,
+                            bdsim_observables* out);
+/* The same for a resident batch's current positions (after the runs queued
+ * on its stream); only the records cross to the host.                        */
+int bdsim_batch_observe(bdsim_batch* b, int force_kind, bdsim_observables* out);
+
 #ifdef __cplusplus
 }
 #endif

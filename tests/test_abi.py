@@ -39,3 +39,4 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.CSim) == 40
     assert ctypes.sizeof(_abi.CCounters) == 10 * 8 + 8 + 8
     assert ctypes.sizeof(_abi.CStepLog) == 4 * 8 + 6 * 4
+    assert ctypes.sizeof(_abi.CObservables) == 26 * 8
