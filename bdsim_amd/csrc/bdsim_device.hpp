@@ -427,6 +427,10 @@ template <typename T>
 using lds_cptr = const __attribute__((address_space(3))) T*;
 template <typename T>
 __device__ __forceinline__ lds_cptr<T> as_lds(const T* p) { return (lds_cptr<T>)p; }
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+template <typename T>
+__device__ __forceinline__ lds_ptr<T> as_lds_mut(T* p) { return (lds_ptr<T>)p; }
 
 // ---- excluded volume (excluded_volume.cpp:36-58) ----------------------------
 // Reciprocal of x > 0: v_rcp_f32 (1 ulp) in FP32; in FP64 v_rcp_f64 refined by

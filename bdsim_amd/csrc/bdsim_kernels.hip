@@ -274,7 +274,206 @@ __device__ __noinline__ D3 lower_matvec_ool(Smem<BLOCK, MIXED>& sm, gbl_ptr<doub
     return y;
 }
 
-template <int BLOCK, bool MIXED>
+// ---- the dense factor held in LDS (Cholesky chains with 3N <= 16 NB) --------
+// The RPY matrix, padded with unit rows to 16 nb (nb = ceil(3N/16) <= NB), is
+// kept as its lower nb(nb+1)/2 blocks of 16 x 16: block (I,J) at
+// 256 (I(I+1)/2 + J), element (r,c) of a block at 16 r + (c ^ r). The XOR puts
+// the 16 lanes of a row-wise or a column-wise access on 16 distinct bank pairs.
+template <int NB>
+struct DenseLds {
+    double blk[NB * (NB + 1) / 2 * 256];
+    double xs[16 * NB];         // operand and result of L x
+};
+
+__device__ __forceinline__ int dl_block(int I, int J) { return (I * (I + 1) / 2 + J) * 256; }
+__device__ __forceinline__ int dl_elem(int r, int c) { return 16 * r + (c ^ r); }
+__device__ __forceinline__ int dl_at(int i, int j) {
+    return dl_block(i >> 4, j >> 4) + dl_elem(i & 15, j & 15);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+// Dense RPY tensor (the entries of dense_factor_ool) and its lower Cholesky
+// factor, in LDS, by 16 x 16 blocks, right-looking: per block column K the
+// diagonal block is factored in registers (lane r holds row r; the pivots and
+// the column broadcast by readlane), the blocks below it are solved against
+// it (lane = row), and every trailing block (I,J) takes -= L_IK L_JK^T on
+// the FP64 matrix cores (v_mfma_f64_16x16x4f64, four per block). Every entry
+// still subtracts its products in increasing k and divides by the pivot
+// (linalg.cpp:91-107); the matrix cores group four products per step. Pivots
+// are taken in column order, so a non-positive one is reported at the same
+// index as the column-by-column factorisation.
+template <int BLOCK, bool MIXED, int NB>
+__device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<double> Lb, int N,
+                                              double hstar) {
+    static_assert(BLOCK == 64, "one wave per chain");
+    const int n = 3 * N, tid = threadIdx.x;
+    const int nb = (n + 15) >> 4;
+    const double RPI = 1.7724538509055160273;
+    const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
+                 C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75;
+    __syncthreads();
+    // padding rows n .. 16 nb - 1: zero, unit diagonal
+    {
+        const int np = 16 * nb, rows = np - n;
+        for (int e = tid; e < rows * np; e += BLOCK) {
+            const int i = n + e / np, j = e % np;
+            if (j <= i) Lb[dl_at(i, j)] = i == j ? 1.0 : 0.0;
+        }
+    }
+    // lower triangle of D: bead mu = lane, beads nu <= mu
+    if (tid < N) {
+        const int mu = tid;
+        for (int nu = 0; nu < mu; ++nu) {
+            double b[3];
+            for (int d = 0; d < 3; ++d) b[d] = sm.R[d][mu] - sm.R[d][nu];
+            double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            if (sq < 1.0e-12) sq = 1.0e-12;
+            const double rs = sqrt(sq);
+            const double hsbyrs = hstar / rs, rsbyhs = rs / hstar;
+            const double hsbyrs2 = hsbyrs * hsbyrs;
+            double om1, om2;
+            if (rsbyhs >= TRPI) {
+                om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
+                om2 = RPI3by4 * (hsbyrs / (rs * rs)) * (1.0 - TPI * hsbyrs2);
+            } else {
+                om1 = 1.0 - C3 * rsbyhs;
+                om2 = C7 * rsbyhs / (rs * rs);
+            }
+            // entry (3mu+p, 3nu+q); the reference's index i is the lower bead's (nu: q)
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q)
+                    Lb[dl_at(3 * mu + p, 3 * nu + q)] = om2 * b[q] * b[p] + (q == p ? om1 : 0.0);
+        }
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q <= p; ++q) Lb[dl_at(3 * mu + p, 3 * mu + q)] = p == q ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int r = tid & 15, g = tid >> 4;
+    for (int K = 0; K < nb; ++K) {
+        // diagonal block: every lane loads row r (lanes 16.. duplicate 0..15)
+        const int dk = dl_block(K, K);
+        double row[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) row[c] = Lb[dk + dl_elem(r, c)];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const double piv = readlane_f64(row[c], c);
+            if (!(piv > 0.0) && 16 * K + c < n) {
+                if (tid == 0) {
+                    sm.cnt.fail_index = 16 * K + c;
+                    sm.cnt.fail_value = piv;
+                }
+                __syncthreads();
+                return false;
+            }
+            const double s = sqrt(piv);
+            row[c] = r == c ? s : row[c] / s;
+#pragma unroll
+            for (int q = c + 1; q < 16; ++q) row[q] -= row[c] * readlane_f64(row[c], q);
+        }
+        if (tid < 16) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c <= r) Lb[dk + dl_elem(r, c)] = row[c];
+        }
+        // blocks below: L_IK = A_IK L_KK^-T, lane = (block I = K+1+g+4m, row r)
+        for (int I0 = K + 1; I0 < nb; I0 += 4) {
+            const int I = I0 + g;
+            const int bi = dl_block(I < nb ? I : K, K);
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = Lb[bi + dl_elem(r, c)];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                double sum = x[c];
+#pragma unroll
+                for (int k = 0; k < c; ++k) sum -= x[k] * readlane_f64(row[k], c);
+                x[c] = sum / readlane_f64(row[c], c);
+            }
+            if (I < nb) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) Lb[bi + dl_elem(r, c)] = x[c];
+            }
+        }
+        __syncthreads();
+        // trailing blocks K < J <= I < nb: A_IJ -= L_IK L_JK^T. Operand
+        // layouts of the 16x16x4 FP64 product: A lane l = (row l%16, k l/16),
+        // B lane l = (k l/16, column l%16), C lane l value q = (row l/16 + 4q,
+        // column l%16).
+        for (int I = K + 1; I < nb; ++I) {
+            const int bIK = dl_block(I, K);
+            for (int J = K + 1; J <= I; ++J) {
+                const int bIJ = dl_block(I, J), bJK = dl_block(J, K);
+                mfma_d4 acc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = Lb[bIJ + dl_elem(g + 4 * q, r)];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double a = -Lb[bIK + dl_elem(r, 4 * kk + g)];
+                    const double bv = Lb[bJK + dl_elem(r, 4 * kk + g)];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Lb[bIJ + dl_elem(g + 4 * q, r)] = acc[q];
+            }
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// y = L x with the factor in LDS (linalg.cpp:109-117: s += L[i][j] x[j] for
+// j = 0..i), one row per lane, then regrouped by bead.
+template <int BLOCK, bool MIXED, int NB>
+__device__ __noinline__ D3 lower_matvec_lds(lds_ptr<double> Lb, lds_ptr<double> xs, int N, double x0,
+                                            double x1, double x2) {
+    static_assert(BLOCK == 64, "one wave per chain");
+    const int tid = threadIdx.x, n = 3 * N;
+    __syncthreads();
+    if (tid < N) {
+        xs[3 * tid] = x0;
+        xs[3 * tid + 1] = x1;
+        xs[3 * tid + 2] = x2;
+    }
+    __syncthreads();
+    double yv[(16 * NB + 63) / 64];
+#pragma unroll
+    for (int pass = 0; pass < (16 * NB + 63) / 64; ++pass) {
+        const int i = tid + 64 * pass;
+        double s = 0.0;
+        if (i < n) {
+            const int I = i >> 4, rr = i & 15;
+            for (int J = 0; J <= I; ++J) {
+                const int base = dl_block(I, J) + 16 * rr;
+                const int cend = J == I ? rr : 15;
+                for (int c = 0; c <= cend; ++c) s += Lb[base + (c ^ rr)] * xs[16 * J + c];
+            }
+        }
+        yv[pass] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int pass = 0; pass < (16 * NB + 63) / 64; ++pass) {
+        const int i = tid + 64 * pass;
+        if (i < n) xs[i] = yv[pass];
+    }
+    __syncthreads();
+    D3 y{{0.0, 0.0, 0.0}};
+    if (tid < N)
+        for (int p = 0; p < 3; ++p) y.v[p] = xs[3 * tid + p];
+    return y;
+}
+
+// NB > 0: the kernel holds a DenseLds<NB> and the Cholesky factor of chains
+// with 3N <= 16 NB lives there instead of in the workspace.
+template <int BLOCK, bool MIXED, int NB = 0>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
     using S = Smem<BLOCK, MIXED>;
@@ -286,12 +485,16 @@ struct Chain {
     const bool hi;
     const WsLayout wl;
     gbl_ptr<double> ws;         // this trajectory's workspace
+    lds_ptr<double> dlb, dxs;   // DenseLds blocks and operand (NB > 0)
     int par = 0;
 
-    __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a)
+    __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
+                     double* dense_blk = nullptr, double* dense_xs = nullptr)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
         ws = as_gbl(A.ws + P.ws_off);
+        dlb = as_lds_mut(dense_blk);
+        dxs = as_lds_mut(dense_xs);
     }
 
     __device__ __forceinline__ double bsum(double v) { return block_sum<NW>(v, sm.Red, par); }
@@ -700,12 +903,21 @@ struct Chain {
     // Dense RPY tensor and its lower Cholesky factor in the workspace
     // (dense_factor_ool); false on a non-positive pivot.
     __device__ __forceinline__ bool dense_factor() {
+        if constexpr (NB > 0) {
+            if (n3 <= 16 * NB) return dense_factor_lds<BLOCK, MIXED, NB>(sm, dlb, N, P.hstar);
+        }
         return dense_factor_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, P.hstar);
     }
 
-    // y = L x with the factor in the workspace (lower_matvec_ool), own rows.
+    // y = L x with the factor where dense_factor left it, own rows.
     __device__ __forceinline__ void lower_matvec(const double x[3], double y[3]) {
-        const D3 r = lower_matvec_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, x[0], x[1], x[2]);
+        D3 r;
+        if constexpr (NB > 0) {
+            if (n3 <= 16 * NB) r = lower_matvec_lds<BLOCK, MIXED, NB>(dlb, dxs, N, x[0], x[1], x[2]);
+            else r = lower_matvec_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, x[0], x[1], x[2]);
+        } else {
+            r = lower_matvec_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, x[0], x[1], x[2]);
+        }
         y[0] = r.v[0]; y[1] = r.v[1]; y[2] = r.v[2];
     }
 
@@ -1157,6 +1369,23 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)
     c.run(ti);
 }
 
+// Chains of the Cholesky method with 3N <= 16 NB: the factor in LDS.
+template <int BLOCK, bool MIXED, int NB>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1))) k_integrate_dense(KArgs a) {
+    __shared__ Smem<BLOCK, MIXED> sm;
+    __shared__ DenseLds<NB> dl;
+    const int ti = a.list[blockIdx.x];
+    const TrajParams* __restrict__ P = a.tp + ti;
+    if (a.seg && a.seg[ti].done) {
+        if (threadIdx.x == 0) a.done_out[blockIdx.x] = 1;
+        return;
+    }
+    init_counters(sm);
+    if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
+    Chain<BLOCK, MIXED, NB> c(sm, *P, a, dl.blk, dl.xs);
+    c.run(ti);
+}
+
 // ---- force / noise / mobility kernels for the piecewise entry points -------
 enum : int { FORCE_TOTAL = 0, FORCE_SPRING = 1, FORCE_BENDING = 2, FORCE_EV = 3 };
 
@@ -1369,12 +1598,16 @@ struct DevBuf {
     }
 };
 
+// kernel buckets of bdsim_integrate_batch: blocks 64..1024, then the two
+// LDS-factor Cholesky kernels
+constexpr int N_BUCKETS = 7;
+
 struct Ctx {
     bool ready = false;
     int device = -1;
     hipStream_t stream = nullptr;
-    hipStream_t bstream[5] = {};        // one per block-size bucket
-    hipEvent_t bevent[5] = {}, uploaded = nullptr;
+    hipStream_t bstream[N_BUCKETS] = {};    // one per kernel bucket
+    hipEvent_t bevent[N_BUCKETS] = {}, uploaded = nullptr;
     uint32_t* jx = nullptr;
     uint32_t* jy = nullptr;
     DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log, obs;
@@ -1407,7 +1640,7 @@ int ensure_init() {
     if (g.device < 0) g.device = 0;
     HIP_TRY(hipSetDevice(g.device));
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-    for (int b = 0; b < 5; ++b) {
+    for (int b = 0; b < N_BUCKETS; ++b) {
         HIP_TRY(hipStreamCreateWithFlags(&g.bstream[b], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&g.bevent[b], hipEventDisableTiming));
     }
@@ -1529,8 +1762,30 @@ int wpe_choice() {
     return v;
 }
 
+// Cholesky chains whose factor fits the LDS of a one-wave workgroup: the
+// DenseLds size (in 16 x 16 blocks per side) of their kernel, or 0.
+// BDSIM_DENSE_LDS=0 in the environment keeps every factor in the workspace.
+int dense_nb_for(int N, double hstar, int hi_method) {
+    static const bool enabled = [] {
+        const char* e = std::getenv("BDSIM_DENSE_LDS");
+        return !(e && e[0] == '0');
+    }();
+    if (!enabled || !(hstar > 0.0) || hi_method != 1) return 0;
+    if (3 * N <= 64) return 4;
+    if (3 * N <= 96) return 6;
+    return 0;
+}
+
 template <bool MIXED>
-void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a) {
+void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, int nb = 0) {
+    if (nb == 4) {
+        hipLaunchKernelGGL((k_integrate_dense<64, MIXED, 4>), grid, dim3(64), 0, stream, a);
+        return;
+    }
+    if (nb == 6) {
+        hipLaunchKernelGGL((k_integrate_dense<64, MIXED, 6>), grid, dim3(64), 0, stream, a);
+        return;
+    }
     const int w = wpe_choice();
     switch (block) {
         case 64:
@@ -1546,9 +1801,9 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a) 
 }
 
 template <bool MIXED>
-int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a) {
+int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a, int nb) {
     if (count == 0) return 0;
-    launch_integrate<MIXED>(block, dim3(count), stream, a);
+    launch_integrate<MIXED>(block, dim3(count), stream, a, nb);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1559,7 +1814,7 @@ T* as(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
 }  // namespace
 
 struct bdsim_batch {
-    int n_traj = 0, N = 0, precision = 0, block = 64;
+    int n_traj = 0, N = 0, precision = 0, block = 64, dense_nb = 0;
     hipStream_t stream = nullptr;
     TrajParams* tp = nullptr;
     int32_t* list = nullptr;
@@ -1616,16 +1871,25 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
     }
     if (blob.empty()) blob.push_back(0.0);
     if (ext.empty()) ext.push_back(ExtEntry{});
-    // Buckets by block size, each launched on its own stream so that buckets
-    // overlap on the device; within a bucket the longest chains go first
-    // (workgroups are dispatched in order, so the short ones fill in behind).
+    // Buckets by kernel (block size; Cholesky chains with the factor in LDS
+    // apart), each launched on its own stream so that buckets overlap on the
+    // device; within a bucket the longest chains go first (workgroups are
+    // dispatched in order, so the short ones fill in behind).
     std::vector<int32_t> order;
-    std::vector<int> counts(5, 0);
-    const int blocks[5] = {64, 128, 256, 512, 1024};
-    for (int b = 0; b < 5; ++b) {
+    std::vector<int> counts(N_BUCKETS, 0);
+    const int blocks[N_BUCKETS] = {64, 128, 256, 512, 1024, 64, 64};
+    const int dnb[N_BUCKETS] = {0, 0, 0, 0, 0, 4, 6};
+    std::vector<int> bucket(n_traj);
+    for (int i = 0; i < n_traj; ++i) {
+        const int nb = dense_nb_for(nbeads[i], phys[i].hstar, phys[i].hi_method);
+        int b = 0;
+        while (!(blocks[b] == block_for(nbeads[i]) && dnb[b] == nb)) ++b;
+        bucket[i] = b;
+    }
+    for (int b = 0; b < N_BUCKETS; ++b) {
         const size_t first = order.size();
         for (int i = 0; i < n_traj; ++i)
-            if (block_for(nbeads[i]) == blocks[b]) { order.push_back(i); ++counts[b]; }
+            if (bucket[i] == b) { order.push_back(i); ++counts[b]; }
         std::stable_sort(order.begin() + first, order.end(),
                          [&](int32_t x, int32_t y) { return nbeads[x] > nbeads[y]; });
     }
@@ -1666,16 +1930,16 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
     a.jx = g.jx;
     a.jy = g.jy;
     a.sim_override = 0;
-    int starts[5];
-    for (int b = 0, st = 0; b < 5; ++b) { starts[b] = st; st += counts[b]; }
+    int starts[N_BUCKETS];
+    for (int b = 0, st = 0; b < N_BUCKETS; ++b) { starts[b] = st; st += counts[b]; }
     HIP_TRY(hipEventRecord(g.uploaded, g.stream));
-    for (int b = 4; b >= 0; --b) {                // largest blocks first
+    for (int b = N_BUCKETS - 1; b >= 0; --b) {    // dense and largest blocks first
         if (counts[b] == 0) continue;
         HIP_TRY(hipStreamWaitEvent(g.bstream[b], g.uploaded, 0));
         a.list = as<int32_t>(g.list) + starts[b];
         int rc = precision == BDSIM_PRECISION_MIXED
-                     ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a)
-                     : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a);
+                     ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a, dnb[b])
+                     : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a, dnb[b]);
         if (rc) {
             (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
             return rc;
@@ -1731,6 +1995,7 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
     b->N = N;
     b->precision = precision;
     b->block = block_for(N);
+    b->dense_nb = dense_nb_for(N, phys->hstar, phys->hi_method);
     auto cleanup = [&](int code, const char* what) {
         bdsim_batch_destroy(b);
         return fail(code, what);
@@ -1779,8 +2044,8 @@ int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
     a.t1 = sim->time_end;
     a.dt = sim->dt;
     a.tol = sim->implicit_loop_tol;
-    if (b->precision == BDSIM_PRECISION_MIXED) launch_integrate<true>(b->block, dim3(b->n_traj), b->stream, a);
-    else launch_integrate<false>(b->block, dim3(b->n_traj), b->stream, a);
+    if (b->precision == BDSIM_PRECISION_MIXED) launch_integrate<true>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb);
+    else launch_integrate<false>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb);
     HIP_TRY(hipGetLastError());
     return 0;
 }

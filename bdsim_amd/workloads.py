@@ -8,6 +8,9 @@ with tau_1 = 1/sin^2(pi/2N), the longest Rouse relaxation time in code units
 
 uneven: 4096 chains with N log-uniform in [10, 256], the headline physics.
 
+cholesky: 8192 chains of N=30 with the dense Cholesky square root (a 90 x 90
+factor per chain and step), Gaussian excluded volume, planar extension.
+
 long: 64 chains of N=1000 with bending and Lennard-Jones excluded volume,
 self-avoiding starts.
 """
@@ -111,3 +114,20 @@ def long_workload(n_traj=64, N=1000, seed=2):
     c.bend_type, c.bend_stiffness = 1, 5.0
     c.hstar = 0.2
     return np.full(n_traj, N, dtype=np.int32), R, [c] * n_traj, 5e-4
+
+
+def cholesky_workload(n_traj=8192, N=30, seed=4):
+    """8192 chains of N=30, FENE b=50, RPY h*=0.25 with the dense Cholesky
+    square root, Gaussian EV z*=1 d*=0.5, planar extension rate 0.5, dt=1e-3.
+    Starts: Gaussian bonds of per-component variance b/(b+5), the second moment
+    of the equilibrium FENE connector, one generator per chain (seed*1e6 + i)."""
+    b = 50.0
+    sd = np.sqrt(b / (b + 5.0))
+    R = [_walk(np.random.default_rng(seed * 1_000_000 + i), N, bond=sd) for i in range(n_traj)]
+    c = headline_phys()
+    c.number_of_beads = N
+    c.sqrtb = np.sqrt(b)
+    c.ev_type, c.zstar, c.dstar = 1, 1.0, 0.5
+    c.hstar, c.hi_method = 0.25, 1
+    c.flow.kappa0[0], c.flow.kappa0[4] = 0.5, -0.5
+    return np.full(n_traj, N, dtype=np.int32), R, [c] * n_traj, 1e-3

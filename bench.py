@@ -43,7 +43,8 @@ sys.path.insert(0, REPO)
 
 from bdsim_amd import _abi as A  # noqa: E402
 from bdsim_amd.workloads import (N_BEADS, N_TRAJ, WI, headline_phys, kappas, shear_rate,  # noqa: E402
-                                 long_workload, start_configs, uneven_workload)
+                                 cholesky_workload, long_workload, start_configs,
+                                 uneven_workload)
 
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}
 
@@ -137,7 +138,8 @@ def run_workload(lib, name, precision, steps, warmup):
     """Integrate a mixed-size workload through bdsim_integrate_batch: one warm-up
     segment, then one timed segment; the time includes the host <-> device
     copies of positions and states (a few MB)."""
-    N, R, phys, dt = uneven_workload() if name == "uneven" else long_workload()
+    N, R, phys, dt = {"uneven": uneven_workload, "long": long_workload,
+                      "cholesky": cholesky_workload}[name]()
     n = len(N)
     Rf = np.ascontiguousarray(np.concatenate([r.ravel() for r in R]))
     ph = (A.CPhys * n)(*phys)
@@ -264,9 +266,11 @@ def main():
     ap.add_argument("--serving-step-scale", type=float, default=0.02)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
-    ap.add_argument("--workload", choices=["headline", "uneven", "long"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "uneven", "long", "cholesky"],
+                    default="headline",
                     help="uneven: 4096 chains with N log-uniform in [10,256]; "
-                         "long: 64 chains of N=1000 (each prints its own line)")
+                         "long: 64 chains of N=1000; cholesky: 8192 chains of N=30 with the "
+                         "dense Cholesky square root (each prints its own line)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -218,6 +218,29 @@ def test_uneven_workload_sample_matches_oracle(gpu, orc):
         assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st.ix, st.iy, st.seed)
 
 
+@pytest.mark.parametrize("N", [2, 5, 16, 21, 22, 27, 32, 33])
+def test_cholesky_factor_sizes(gpu, orc, N):
+    """The Cholesky method across the sizes whose factor is held in LDS (3N up
+    to 64 and up to 96: padding rows, whole padding blocks, exact fit) and just
+    beyond (N=33, factor in the workspace): 30 steps within 1e-10 of the FP64
+    oracle with the same decisions; the mixed form within 1e-5."""
+    d = C.phys_dict(spring=2, sqrtb=np.sqrt(50.0), hstar=0.25, method=1, ev=1, zstar=1.0,
+                    dstar=0.5, kappa=[0.5, 0, 0, 0, -0.5, 0, 0, 0, 0])
+    p = H.Packed(d, N)
+    R0 = C.random_walk(N, 300 + N, 0.95)
+    sim = H.csim(C.sim_dict(0.0, 0.029, 1e-3, 1e-3))
+    want, rc, cnto, lo = H.oracle_integrate(orc, N, R0, p.c, sim, H.rng_of(N), log_cap=32)
+    assert rc == 0 and cnto.steps == 30
+    out, rngs, status, cnts, logs = H.gpu_batch(gpu, [(N, R0, p.c, sim, H.rng_of(N))], log_stride=32)
+    assert status[0] == 0
+    assert rel(out[0], want) <= 1e-10, rel(out[0], want)
+    for k in range(30):
+        assert logs[k].corrector_iters == lo[k].corrector_iters, k
+    out, _, status, _, _ = H.gpu_batch(gpu, [(N, R0, p.c, sim, H.rng_of(N))],
+                                       precision=A.PRECISION_MIXED)
+    assert status[0] == 0 and rel(out[0], want) < 1e-5
+
+
 def test_long_chains_n1000(gpu, orc):
     """Two chains of the N=1000 workload (FENE, bending, Lennard-Jones, RPY
     h*=0.2 Chebyshev, 16 waves per chain): 6 steps within 1e-10 of the FP64
