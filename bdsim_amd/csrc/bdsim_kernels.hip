@@ -302,13 +302,13 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 // Dense RPY tensor (the entries of dense_factor_ool) and its lower Cholesky
 // factor, in LDS, by 16 x 16 blocks, right-looking: per block column K the
 // diagonal block is factored in registers (lane r holds row r; the pivots and
-// the column broadcast by readlane), the blocks below it are solved against
-// it (lane = row), and every trailing block (I,J) takes -= L_IK L_JK^T on
-// the FP64 matrix cores (v_mfma_f64_16x16x4f64, four per block). Every entry
-// still subtracts its products in increasing k and divides by the pivot
-// (linalg.cpp:91-107); the matrix cores group four products per step. Pivots
-// are taken in column order, so a non-positive one is reported at the same
-// index as the column-by-column factorisation.
+// the column broadcast by readlane; each entry subtracts its products in
+// increasing k and divides by the pivot, linalg.cpp:91-107), the blocks below
+// it are multiplied by the inverse of that factor, and every trailing block
+// (I,J) takes -= L_IK L_JK^T, both on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64, four per block). The result equals the
+// column-by-column factor up to rounding. Pivots are taken in column order,
+// so a non-positive one is reported at the same index.
 template <int BLOCK, bool MIXED, int NB>
 __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<double> Lb, int N,
                                               double hstar) {
@@ -327,10 +327,12 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             if (j <= i) Lb[dl_at(i, j)] = i == j ? 1.0 : 0.0;
         }
     }
-    // lower triangle of D: bead mu = lane, beads nu <= mu
-    if (tid < N) {
-        const int mu = tid;
-        for (int nu = 0; nu < mu; ++nu) {
+    // lower triangle of D: bead pairs mu > nu spread over the wave (pair
+    // index p = mu(mu-1)/2 + nu, lane l takes p = l, l + 64, ...)
+    {
+        int mu = 1, nu = tid;
+        while (nu >= mu) { nu -= mu; ++mu; }
+        for (; mu < N;) {
             double b[3];
             for (int d = 0; d < 3; ++d) b[d] = sm.R[d][mu] - sm.R[d][nu];
             double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
@@ -350,9 +352,12 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q)
                     Lb[dl_at(3 * mu + p, 3 * nu + q)] = om2 * b[q] * b[p] + (q == p ? om1 : 0.0);
+            nu += BLOCK;
+            while (nu >= mu) { nu -= mu; ++mu; }
         }
-        for (int p = 0; p < 3; ++p)
-            for (int q = 0; q <= p; ++q) Lb[dl_at(3 * mu + p, 3 * mu + q)] = p == q ? 1.0 : 0.0;
+        if (tid < N)
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q <= p; ++q) Lb[dl_at(3 * tid + p, 3 * tid + q)] = p == q ? 1.0 : 0.0;
     }
     __syncthreads();
     const int r = tid & 15, g = tid >> 4;
@@ -364,8 +369,9 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
         for (int c = 0; c < 16; ++c) row[c] = Lb[dk + dl_elem(r, c)];
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
+            if (16 * K + c >= n) break;      // padding: already unit rows
             const double piv = readlane_f64(row[c], c);
-            if (!(piv > 0.0) && 16 * K + c < n) {
+            if (!(piv > 0.0)) {
                 if (tid == 0) {
                     sm.cnt.fail_index = 16 * K + c;
                     sm.cnt.fail_value = piv;
@@ -383,23 +389,35 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             for (int c = 0; c < 16; ++c)
                 if (c <= r) Lb[dk + dl_elem(r, c)] = row[c];
         }
-        // blocks below: L_IK = A_IK L_KK^-T, lane = (block I = K+1+g+4m, row r)
-        for (int I0 = K + 1; I0 < nb; I0 += 4) {
-            const int I = I0 + g;
-            const int bi = dl_block(I < nb ? I : K, K);
-            double x[16];
+        // blocks below: L_IK = A_IK L_KK^-T. Lane r forms row r of the
+        // inverse of the diagonal factor (x L = e_r, solved from the last
+        // column down), then every block below is one 16x16x16 product on the
+        // matrix cores; the B operand of lane group g is column 4kk+g of that
+        // row, picked from registers.
+        {
+            double dinv[16], xr[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) x[c] = Lb[bi + dl_elem(r, c)];
+            for (int j = 0; j < 16; ++j) dinv[j] = 1.0 / readlane_f64(row[j], j);
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                double sum = x[c];
+            for (int j = 15; j >= 0; --j) {
+                double sum = j == r ? 1.0 : 0.0;
 #pragma unroll
-                for (int k = 0; k < c; ++k) sum -= x[k] * readlane_f64(row[k], c);
-                x[c] = sum / readlane_f64(row[c], c);
+                for (int k = j + 1; k < 16; ++k) sum -= xr[k] * readlane_f64(row[j], k);
+                xr[j] = sum * dinv[j];
             }
-            if (I < nb) {
+            double bop[4];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) Lb[bi + dl_elem(r, c)] = x[c];
+            for (int kk = 0; kk < 4; ++kk)
+                bop[kk] = g == 0 ? xr[4 * kk] : g == 1 ? xr[4 * kk + 1] : g == 2 ? xr[4 * kk + 2] : xr[4 * kk + 3];
+            for (int I = K + 1; I < nb; ++I) {
+                const int bi = dl_block(I, K);
+                mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lb[bi + dl_elem(r, 4 * kk + g)], bop[kk],
+                                                               acc, 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Lb[bi + dl_elem(g + 4 * q, r)] = acc[q];
             }
         }
         __syncthreads();
