@@ -157,9 +157,15 @@ def run_workload(lib, name, precision, steps, warmup):
     if warmup > 0:
         segment(0.0, warmup)
     wall = segment(warmup * dt, steps)
-    mv = sum(int(c.matvecs) * 18.0 * int(N[i]) ** 2 for i, c in enumerate(cnt))
+    # algorithmic FLOPs: 18 N^2 per D.x the run counted, plus n^3/3 + 2 n^2 per dense
+    # factor and L x (n = 3N): one per step for the Cholesky method, one per fallback
+    fl = 0.0
+    for i, c in enumerate(cnt):
+        nn = 3.0 * int(N[i])
+        factors = int(c.fallbacks) + (int(c.steps) if phys[i].hi_method == 1 and phys[i].hstar > 0 else 0)
+        fl += int(c.matvecs) * 18.0 * int(N[i]) ** 2 + factors * (nn ** 3 / 3.0 + 2.0 * nn * nn)
     return dict(bead_steps_per_s=float(N.sum()) * steps / wall, wall_s=wall, n_traj=n,
-                beads=int(N.sum()), status_ok=int(np.sum(st == 0)), tflops=mv / wall / 1e12,
+                beads=int(N.sum()), status_ok=int(np.sum(st == 0)), tflops=fl / wall / 1e12,
                 finite=bool(np.all(np.isfinite(Rf))))
 
 
