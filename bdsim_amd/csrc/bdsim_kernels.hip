@@ -83,21 +83,10 @@ __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// hydrodynamics.cpp:91-101
-__device__ inline int sturm_count(int k, const double* a, const double* b, double x) {
-    int count = 0;
-    double d = a[0] - x;
-    if (d < 0.0) ++count;
-    for (int i = 1; i < k; ++i) {
-        if (d == 0.0) d = 1.0e-300;
-        d = (a[i] - x) - b[i - 1] * b[i - 1] / d;
-        if (d < 0.0) ++count;
-    }
-    return count;
-}
-
 // hydrodynamics.cpp:106-136; c, dd, y are this lane's rows of the workspace.
-__device__ inline void tridiag_eigenvector(int m, const double* a, const double* b, double lambda,
+// The recurrences carry c[i-1], dd[i-1] and y[i+1] in registers (the same
+// operations; the rows are kept for the back substitution and the result).
+__device__ inline void tridiag_eigenvector(int m, lds_cptr<double> a, lds_cptr<double> b, double lambda,
                                            gbl_ptr<double> c, gbl_ptr<double> dd, gbl_ptr<double> y) {
     const double scale = (fabs(lambda) < 1.0) ? 1.0 : fabs(lambda);
     const double shift = lambda - 1.0e-7 * scale;
@@ -105,16 +94,27 @@ __device__ inline void tridiag_eigenvector(int m, const double* a, const double*
     for (int it = 0; it < 2; ++it) {
         double beta0 = a[0] - shift;
         if (fabs(beta0) < 1.0e-300) beta0 = 1.0e-300;
-        c[0] = (m > 1 ? b[0] / beta0 : 0.0);
-        dd[0] = y[0] / beta0;
+        double cp = (m > 1 ? b[0] / beta0 : 0.0);
+        double dp = y[0] / beta0;
+        c[0] = cp;
+        dd[0] = dp;
         for (int i = 1; i < m; ++i) {
-            double beta = (a[i] - shift) - b[i - 1] * c[i - 1];
+            const double bi1 = b[i - 1];
+            double beta = (a[i] - shift) - bi1 * cp;
             if (fabs(beta) < 1.0e-300) beta = 1.0e-300;
-            c[i] = (i < m - 1 ? b[i] / beta : 0.0);
-            dd[i] = (y[i] - b[i - 1] * dd[i - 1]) / beta;
+            const double ci = (i < m - 1 ? b[i] / beta : 0.0);
+            const double di = (y[i] - bi1 * dp) / beta;
+            c[i] = ci;
+            dd[i] = di;
+            cp = ci;
+            dp = di;
         }
-        y[m - 1] = dd[m - 1];
-        for (int i = m - 2; i >= 0; --i) y[i] = dd[i] - c[i] * y[i + 1];
+        double yn = dp;
+        y[m - 1] = yn;
+        for (int i = m - 2; i >= 0; --i) {
+            yn = dd[i] - c[i] * yn;
+            y[i] = yn;
+        }
         double nrm = 0.0;
         for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
         nrm = sqrt(nrm);
@@ -134,7 +134,17 @@ __device__ inline void tridiag_eigenvector(int m, const double* a, const double*
 // first reaches the target, until the bracket stops shrinking (floating-point
 // resolution); the value is its midpoint, as in the reference. Results:
 // sm_out[0] = smallest, sm_out[1] = largest.
-__device__ __noinline__ void ritz_multisect(int k, const double* al, const double* be,
+template <typename T>
+__device__ __forceinline__ T sturm_div(T num, T d) { return num / d; }
+template <>
+__device__ __forceinline__ float sturm_div<float>(float num, float d) { return num * __builtin_amdgcn_rcpf(d); }
+
+// T = double: the FP64 form's values. T = float (mixed form): the same
+// multisection in FP32 with a reciprocal for the division -- the bounds are
+// padded by 0.5 and 1.02 before use, so FP32 resolution is ample -- in about
+// 5 rounds instead of 11.
+template <typename T>
+__device__ __noinline__ void ritz_multisect(int k, lds_cptr<double> al, lds_cptr<double> be,
                                             double* sm_out) {
     const int lane = threadIdx.x & 63, half = lane >> 5, j = lane & 31;
     double lo = al[0], hi = al[0];
@@ -144,21 +154,32 @@ __device__ __noinline__ void ritz_multisect(int k, const double* al, const doubl
         if (hi < al[i] + rad) hi = al[i] + rad;
     }
     const int target = half == 0 ? 1 : k;
-    double a = lo, b = hi;
+    T a = (T)lo, b = (T)hi;
     for (int round = 0; round < 64; ++round) {
-        const double w = (b - a) / 33.0;
-        const double x = a + (j + 1) * w;
-        const bool ok = sturm_count(k, al, be, x) >= target;
+        const T w = (b - a) / (T)33.0;
+        const T x = a + (T)(j + 1) * w;
+        int count = 0;
+        {
+            T d = (T)al[0] - x;
+            if (d < (T)0) ++count;
+            for (int i = 1; i < k; ++i) {
+                if (d == (T)0) d = sizeof(T) == 8 ? (T)1.0e-300 : (T)1.0e-30f;
+                const T bi = (T)be[i - 1];
+                d = ((T)al[i] - x) - sturm_div<T>(bi * bi, d);
+                if (d < (T)0) ++count;
+            }
+        }
+        const bool ok = count >= target;
         const uint64_t m = __ballot(ok);
         const uint32_t hm = (uint32_t)(half ? (m >> 32) : m);
         const int f = hm ? __ffs(hm) - 1 : 32;
-        const double nb = f < 32 ? a + (f + 1) * w : b;
-        const double na = f > 0 ? a + f * w : a;
+        const T nb = f < 32 ? a + (T)(f + 1) * w : b;
+        const T na = f > 0 ? a + (T)f * w : a;
         const bool stuck = (nb == b && na == a) || !(na < nb);
         if (!stuck) { a = na; b = nb; }
         if (__all(stuck)) break;
     }
-    if (j == 0) sm_out[half] = 0.5 * (a + b);
+    if (j == 0) sm_out[half] = 0.5 * ((double)a + (double)b);
 }
 
 // spring.cpp:174-181 out of line: the cubic / rtsafe solvers are large and run
@@ -1006,14 +1027,17 @@ struct Chain {
             return k;
         }
         const gbl_ptr<double> tri = ws + wl.tri;
-        if (tid < 64) ritz_multisect(k, sm.Alpha, sm.Beta, sm.Scal);
+        if (tid < 64) {
+            if constexpr (MIXED) ritz_multisect<float>(k, as_lds(sm.Alpha), as_lds(sm.Beta), sm.Scal);
+            else ritz_multisect<double>(k, as_lds(sm.Alpha), as_lds(sm.Beta), sm.Scal);
+        }
         __syncthreads();
         rmin = sm.Scal[0];
         rmax = sm.Scal[1];
         // Ritz vectors of the two extremes by inverse iteration (lanes 0, 1)
         if (tid < 2) {
             const gbl_ptr<double> rows = tri + tid * 3 * LANCZOS_COLD;
-            tridiag_eigenvector(k, sm.Alpha, sm.Beta, tid == 0 ? rmin : rmax, rows,
+            tridiag_eigenvector(k, as_lds(sm.Alpha), as_lds(sm.Beta), tid == 0 ? rmin : rmax, rows,
                                 rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
         }
         __syncthreads();
