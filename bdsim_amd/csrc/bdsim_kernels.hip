@@ -1065,14 +1065,36 @@ struct Chain {
         const gbl_ptr<double> coef = ws + wl.coef;
         const gbl_ptr<double> aux = ws + wl.aux;
         __syncthreads();
-        for (int k = tid; k <= L; k += BLOCK) aux[k] = sqrt(cos(PI * (k + 0.5) / (L + 1)) / da - db / da);
-        __syncthreads();
-        for (int j = tid; j <= L; j += BLOCK) {
-            double s = 0.0;
-            for (int k = 0; k <= L; ++k) s += aux[k] * cos(j * (k + 0.5) * PI / (L + 1));
-            s *= 2.0 / (L + 1);
-            if (j == 0) s /= 2.0;
-            coef[j] = s;
+        if constexpr (MIXED) {
+            // FP32 nodes and sums: cos(pi m / (2(L+1))) = cos(2 pi t) with
+            // t = (m mod 4(L+1)) / (4(L+1)) reduced exactly in integers, on
+            // the hardware cosine (argument in turns)
+            const int per = 4 * (L + 1);
+            const float inv = 1.0f / (float)per;
+            const float fda = (float)da, fdb = (float)db;
+            for (int k = tid; k <= L; k += BLOCK) {
+                const float c = __builtin_amdgcn_cosf((float)(2 * k + 1) * inv);
+                aux[k] = (double)sqrtf(c / fda - fdb / fda);
+            }
+            __syncthreads();
+            for (int j = tid; j <= L; j += BLOCK) {
+                float s = 0.0f;
+                for (int k = 0; k <= L; ++k)
+                    s += (float)aux[k] * __builtin_amdgcn_cosf((float)((j * (2 * k + 1)) % per) * inv);
+                s *= 2.0f / (float)(L + 1);
+                if (j == 0) s *= 0.5f;
+                coef[j] = (double)s;
+            }
+        } else {
+            for (int k = tid; k <= L; k += BLOCK) aux[k] = sqrt(cos(PI * (k + 0.5) / (L + 1)) / da - db / da);
+            __syncthreads();
+            for (int j = tid; j <= L; j += BLOCK) {
+                double s = 0.0;
+                for (int k = 0; k <= L; ++k) s += aux[k] * cos(j * (k + 0.5) * PI / (L + 1));
+                s *= 2.0 / (L + 1);
+                if (j == 0) s /= 2.0;
+                coef[j] = s;
+            }
         }
         __syncthreads();
     }
