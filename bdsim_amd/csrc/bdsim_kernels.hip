@@ -83,11 +83,11 @@ __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// hydrodynamics.cpp:106-136; c, dd, y are this lane's rows of the workspace.
-// The recurrences carry c[i-1], dd[i-1] and y[i+1] in registers (the same
+// hydrodynamics.cpp:106-136; c, dd, y are this lane's rows (in LDS). The
+// recurrences carry c[i-1], dd[i-1] and y[i+1] in registers (the same
 // operations; the rows are kept for the back substitution and the result).
 __device__ inline void tridiag_eigenvector(int m, lds_cptr<double> a, lds_cptr<double> b, double lambda,
-                                           gbl_ptr<double> c, gbl_ptr<double> dd, gbl_ptr<double> y) {
+                                           lds_ptr<double> c, lds_ptr<double> dd, lds_ptr<double> y) {
     const double scale = (fabs(lambda) < 1.0) ? 1.0 : fabs(lambda);
     const double shift = lambda - 1.0e-7 * scale;
     for (int i = 0; i < m; ++i) y[i] = 1.0 / sqrt((double)m);
@@ -1026,7 +1026,9 @@ struct Chain {
             }
             return k;
         }
-        const gbl_ptr<double> tri = ws + wl.tri;
+        // rows c, dd, y of the two inverse iterations: the V area is free here
+        static_assert(sizeof(sm.V) >= 6 * LANCZOS_COLD * sizeof(double), "tridiagonal rows in V");
+        const lds_ptr<double> tri = as_lds_mut(&sm.V[0][0]);
         if (tid < 64) {
             if constexpr (MIXED) ritz_multisect<float>(k, as_lds(sm.Alpha), as_lds(sm.Beta), sm.Scal);
             else ritz_multisect<double>(k, as_lds(sm.Alpha), as_lds(sm.Beta), sm.Scal);
@@ -1036,7 +1038,7 @@ struct Chain {
         rmax = sm.Scal[1];
         // Ritz vectors of the two extremes by inverse iteration (lanes 0, 1)
         if (tid < 2) {
-            const gbl_ptr<double> rows = tri + tid * 3 * LANCZOS_COLD;
+            const lds_ptr<double> rows = tri + tid * 3 * LANCZOS_COLD;
             tridiag_eigenvector(k, as_lds(sm.Alpha), as_lds(sm.Beta), tid == 0 ? rmin : rmax, rows,
                                 rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
         }
@@ -1044,8 +1046,8 @@ struct Chain {
         // lift the tridiagonal eigenvectors: v = V^T y (hydrodynamics.cpp:264-273)
         double vmin[3] = {0.0, 0.0, 0.0}, vmax[3] = {0.0, 0.0, 0.0};
         if (own) {
-            const gbl_ptr<double> ymn = tri + 2 * LANCZOS_COLD;
-            const gbl_ptr<double> ymx = tri + 3 * LANCZOS_COLD + 2 * LANCZOS_COLD;
+            const lds_cptr<double> ymn = tri + 2 * LANCZOS_COLD;
+            const lds_cptr<double> ymx = tri + 3 * LANCZOS_COLD + 2 * LANCZOS_COLD;
             for (int j = 0; j < k; ++j) {
                 const double a0 = ymn[j], a1 = ymx[j];
                 for (int d = 0; d < 3; ++d) {
