@@ -1819,6 +1819,18 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
 // Register budget of the integrate kernel for 64-thread workgroups: waves per
 // SIMD the allocation is held to (BDSIM_WPE overrides; 1, 2 or 4; default 4,
 // the fastest measured at 4096 x N=50).
+// More waves than 3 per SIMD of the device: the launch would not be resident
+// at once with the unconstrained register allocation.
+bool crowded(long long waves) {
+    static const int simds = [] {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        return 4 * (cus > 0 ? cus : 256);
+    }();
+    return waves > 3LL * simds;
+}
+
 int wpe_choice() {
     static int v = [] {
         const char* e = std::getenv("BDSIM_WPE");
@@ -1842,8 +1854,12 @@ int dense_nb_for(int N, double hstar, int hi_method) {
     return 0;
 }
 
+// waves: the waves of everything launched together with this grid (buckets
+// side by side), or -1 for this grid alone.
 template <bool MIXED>
-void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, int nb = 0) {
+void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, int nb = 0,
+                      long long waves = -1) {
+    if (waves < 0) waves = (long long)grid.x * (block / 64);
     if (nb == 4) {
         hipLaunchKernelGGL((k_integrate_dense<64, MIXED, 4>), grid, dim3(64), 0, stream, a);
         return;
@@ -1859,17 +1875,30 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
             else if (w == 2) hipLaunchKernelGGL((k_integrate<64, MIXED, 2>), grid, dim3(64), 0, stream, a);
             else hipLaunchKernelGGL((k_integrate<64, MIXED, 1>), grid, dim3(64), 0, stream, a);
             break;
-        case 128: hipLaunchKernelGGL((k_integrate<128, MIXED, 1>), grid, dim3(128), 0, stream, a); break;
-        case 256: hipLaunchKernelGGL((k_integrate<256, MIXED, 1>), grid, dim3(256), 0, stream, a); break;
-        case 512: hipLaunchKernelGGL((k_integrate<512, MIXED, 1>), grid, dim3(512), 0, stream, a); break;
+        // Multi-wave chains: the register budget of 4 waves per SIMD when the
+        // launch has more waves than 3 per SIMD could hold (all chains then
+        // stay resident; measured 2x on 4096 chains of N in [10, 256]),
+        // otherwise the unconstrained build.
+        case 128:
+            if (crowded(waves)) hipLaunchKernelGGL((k_integrate<128, MIXED, 4>), grid, dim3(128), 0, stream, a);
+            else hipLaunchKernelGGL((k_integrate<128, MIXED, 1>), grid, dim3(128), 0, stream, a);
+            break;
+        case 256:
+            if (crowded(waves)) hipLaunchKernelGGL((k_integrate<256, MIXED, 4>), grid, dim3(256), 0, stream, a);
+            else hipLaunchKernelGGL((k_integrate<256, MIXED, 1>), grid, dim3(256), 0, stream, a);
+            break;
+        case 512:
+            if (crowded(waves)) hipLaunchKernelGGL((k_integrate<512, MIXED, 4>), grid, dim3(512), 0, stream, a);
+            else hipLaunchKernelGGL((k_integrate<512, MIXED, 1>), grid, dim3(512), 0, stream, a);
+            break;
         default: hipLaunchKernelGGL((k_integrate<1024, MIXED, 1>), grid, dim3(1024), 0, stream, a); break;
     }
 }
 
 template <bool MIXED>
-int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a, int nb) {
+int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a, int nb, long long waves) {
     if (count == 0) return 0;
-    launch_integrate<MIXED>(block, dim3(count), stream, a, nb);
+    launch_integrate<MIXED>(block, dim3(count), stream, a, nb, waves);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1997,15 +2026,20 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
     a.jy = g.jy;
     a.sim_override = 0;
     int starts[N_BUCKETS];
-    for (int b = 0, st = 0; b < N_BUCKETS; ++b) { starts[b] = st; st += counts[b]; }
+    long long waves = 0;
+    for (int b = 0, st = 0; b < N_BUCKETS; ++b) {
+        starts[b] = st;
+        st += counts[b];
+        waves += (long long)counts[b] * (blocks[b] / 64);
+    }
     HIP_TRY(hipEventRecord(g.uploaded, g.stream));
     for (int b = N_BUCKETS - 1; b >= 0; --b) {    // dense and largest blocks first
         if (counts[b] == 0) continue;
         HIP_TRY(hipStreamWaitEvent(g.bstream[b], g.uploaded, 0));
         a.list = as<int32_t>(g.list) + starts[b];
         int rc = precision == BDSIM_PRECISION_MIXED
-                     ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a, dnb[b])
-                     : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a, dnb[b]);
+                     ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves)
+                     : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves);
         if (rc) {
             (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
             return rc;

@@ -325,8 +325,13 @@ int srv_service(bdsim_server* s, int gi) {
     a.seg = s->seg;
     a.done_out = gr.d_done;
     a.max_steps = gr.steps;
-    if (s->precision == BDSIM_PRECISION_MIXED) launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a);
-    else launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a);
+    // waves of all groups' active trajectories: the groups run side by side
+    long long waves = 0;
+    for (const SrvGroup& o : s->grp) waves += (long long)o.active.size() * (o.block / 64);
+    if (s->precision == BDSIM_PRECISION_MIXED)
+        launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, 0, waves);
+    else
+        launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a, 0, waves);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(gr.ev, gr.stream));
     gr.in_flight = true;
