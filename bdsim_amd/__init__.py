@@ -7,7 +7,7 @@ The `_bdsim`-compatible surface (parameter types, Rng, integrate, forces) is in
 from .core import (Spring, EV, Bending, DelSMethod, SpringParams, EVParams, BendingParams,
                    PhysParams, SimParams, Flow, Rng, ExternalForce, integrate, spring_force,
                    bending_force, ev_force, total_force, external_force)
-from . import core, flows, initial, properties, storage, parallel, ensemble
+from . import core, flows, initial, properties, storage, parallel, ensemble, resident, rheology
 from ._abi import PRECISION_FP64, PRECISION_MIXED
 from .ensemble import (Initial, Output, INITIALIZERS, n_steps_of, simulate, run_ensemble,
                        trajectory_samples, final_state, sampled_states, mean_stderr,

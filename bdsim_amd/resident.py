@@ -2,9 +2,11 @@
 
 `ResidentEnsemble` holds positions, RNG states and workspace on the device
 (bdsim_batch_*). `run(k)` integrates k steps as one segment [t, t + (k-1) dt],
-the chunking of python/bdsim/ensemble.py:73-86: chunked stepping equals one
+the chunking of python/bdsim/ensemble.py:106-123: chunked stepping equals one
 continuous run except for the cold spectral start each segment makes, exactly
-as successive `integrate` calls do. `observe()` evaluates the snapshot
+as successive `integrate` calls do. `run_segment(t0, t1)` integrates the segment
+[t0, t1] as `integrate` does with `_segment(sim, t0, t1)` (the sampling of
+`trajectory_samples`, python/bdsim/ensemble.py:126-138). `observe()` evaluates the snapshot
 observables of python/bdsim/properties.py:5-52 on the device, so ensemble
 averages (Rg^2, Q^2, the Kramers stress behind the viscosities) need no copy
 of the positions to the host.
@@ -55,7 +57,7 @@ class ResidentEnsemble:
             raise ValueError("R0 must have shape (n_traj, N, 3)")
         self.n, self.N = int(R0.shape[0]), int(R0.shape[1])
         self.sim = sim
-        self.step = 0
+        self.t = float(sim.time_start)
         self._phys = _abi.PackedPhys(phys, self.N)
         K = None
         if kappas is not None:
@@ -70,17 +72,25 @@ class ResidentEnsemble:
 
     @property
     def time(self):
-        return self.sim.time_start + self.step * self.sim.dt
+        return self.t
+
+    def _queue(self, t0, t1):
+        s = _abi.CSim(float(t0), float(t1), float(self.sim.dt), float(self.sim.implicit_loop_tol),
+                      1 if self.sim.update_center_of_mass else 0)
+        _abi.check(self.lib, self.lib.bdsim_batch_run(self.h, byref(s)))
 
     def run(self, n_steps):
         """Queue n_steps steps of every chain as one segment (returns at once)."""
         if n_steps <= 0:
             return
-        t0 = self.time
-        s = _abi.CSim(t0, t0 + (n_steps - 1) * self.sim.dt, float(self.sim.dt),
-                      float(self.sim.implicit_loop_tol), 1 if self.sim.update_center_of_mass else 0)
-        _abi.check(self.lib, self.lib.bdsim_batch_run(self.h, byref(s)))
-        self.step += n_steps
+        self._queue(self.t, self.t + (n_steps - 1) * self.sim.dt)
+        self.t += n_steps * self.sim.dt
+
+    def run_segment(self, t0, t1):
+        """Queue the segment [t0, t1] of every chain (the steps t = t0, t0 + dt,
+        ... while t <= t1 + dt/2, as `integrate` takes a SimParams)."""
+        self._queue(t0, t1)
+        self.t = float(t1)
 
     def observe(self, force="total"):
         """Snapshot observables of every chain at the current step (dict of arrays)."""
