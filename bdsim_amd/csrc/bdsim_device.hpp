@@ -98,22 +98,30 @@ __host__ __device__ inline WsLayout ws_layout(int N, bool hi) {
 // and the tree is fixed, so the result is deterministic run to run.
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    // every pattern used here is a permutation within a row of 16 lanes, so
+    // each lane has a valid source and no previous value is kept
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
-// All-lanes sum: quad swaps and row mirrors by DPP, then lane xor 16 / 32.
-// Each stage adds two operands that are the same pair on both partner lanes,
-// so every lane holds bitwise the same total, and the order is fixed.
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// All-lanes sum: quad swaps and row mirrors by DPP give every lane its row's
+// sum (each stage adds two operands that are the same pair on both partner
+// lanes), then the four row sums are read to scalars and added as
+// (r0 + r1) + (r2 + r3): every lane holds bitwise the same total, and the
+// order is fixed.
 __device__ __forceinline__ double wave_sum(double v) {
     v += dpp_mov<0xB1>(v);     // quad_perm [1,0,3,2]: lane ^ 1
     v += dpp_mov<0x4E>(v);     // quad_perm [2,3,0,1]: lane ^ 2
     v += dpp_mov<0x141>(v);    // row_half_mirror: sums of 8
     v += dpp_mov<0x140>(v);    // row_mirror: sums of 16
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 // Block-wide sum over NW waves. sRed holds 2*NW doubles; `par` alternates the
