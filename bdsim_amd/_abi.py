@@ -76,6 +76,7 @@ STATUS_RTSAFE_MAXIT = 3
 STATUS_EXACT_SQRT = 4
 STATUS_WLCB_SIGMA = 5
 STATUS_NOT_RUN = 6
+STATUS_SYNC_TIMEOUT = 7
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1

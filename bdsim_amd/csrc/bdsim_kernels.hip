@@ -52,6 +52,11 @@ struct KArgs {
     SegState* seg;              // per trajectory
     int32_t* done_out;          // per workgroup: 1 once its segment has ended
     int32_t max_steps;          // steps per launch at most
+    // chains split over `split` workgroups (split > 1): per launch slot a
+    // barrier counter and two exchange buffers of [split][3][1024] doubles
+    int32_t split;
+    uint32_t* bar;
+    double* xbuf;
 };
 
 template <int BLOCK, bool MIXED>
@@ -510,9 +515,30 @@ __device__ __noinline__ D3 lower_matvec_lds(lds_ptr<double> Lb, lds_ptr<double> 
     return y;
 }
 
+// Barrier of the parts of one split chain (thread 0 of each part): count in,
+// then wait until the count reaches `target`. Bounded: false after about a
+// second, so that a part that never arrives cannot hold the others forever.
+__device__ __noinline__ bool chain_barrier(uint32_t* bar, uint32_t target) {
+    __threadfence();
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int it = 0; it < (1 << 20); ++it) {
+        if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+            __threadfence();
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
 // NB > 0: the kernel holds a DenseLds<NB> and the Cholesky factor of chains
 // with 3N <= 16 NB lives there instead of in the workspace.
-template <int BLOCK, bool MIXED, int NB = 0>
+// SPLIT: one of A.split workgroups of the chain (part). Every part runs the
+// whole step on its own copy of the state (the same values: the same inputs
+// and the same operations), except that each D.x sums only its share of the
+// partners; the parts' partial rows meet in HBM behind a per-chain barrier
+// and every part adds them in part order. Part 0 writes the results.
+template <int BLOCK, bool MIXED, int NB = 0, bool SPLIT = false>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
     using S = Smem<BLOCK, MIXED>;
@@ -526,6 +552,9 @@ struct Chain {
     gbl_ptr<double> ws;         // this trajectory's workspace
     lds_ptr<double> dlb, dxs;   // DenseLds blocks and operand (NB > 0)
     int par = 0;
+    int part = 0, nparts = 1;   // SPLIT: this workgroup's share of the chain
+    uint32_t epoch = 0;         // SPLIT: exchanges done
+    bool sync_fail = false;     // SPLIT: a barrier timed out
 
     __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
                      double* dense_blk = nullptr, double* dense_xs = nullptr)
@@ -568,6 +597,61 @@ struct Chain {
         }
     }
 
+    // SPLIT: this workgroup is part p of G of its chain; its state copy lives
+    // in its own workspace
+    __device__ void set_part(int p, int G) {
+        part = p;
+        nparts = G;
+        ws = as_gbl(A.ws + P.ws_off + (int64_t)p * wl.total);
+    }
+
+    // SPLIT: the parts' partial rows (own lanes) go to this launch slot's
+    // exchange buffer (two, alternating), the parts meet at the chain's
+    // barrier, and each part sums the partials in part order -- the same
+    // value in every part. A barrier that times out marks the chain failed
+    // (every later exchange is skipped, so every part ends).
+    // The parts of a chain run on different XCDs, whose L2 caches are not
+    // coherent with each other: the partials are written and read with
+    // device-scope accesses, which bypass them.
+    __device__ static void xput(double* p, double v) {
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ static double xget(double* p) {
+        return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<unsigned long long*>(p),
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+
+    __device__ void exchange(double& ax, double& ay, double& az) {
+        const int slot = blockIdx.x / nparts;
+        double* xb = A.xbuf + ((size_t)slot * 2 + (epoch & 1)) * (size_t)nparts * 3 * 1024;
+        if (own) {
+            xput(xb + (part * 3 + 0) * 1024 + tid, ax);
+            xput(xb + (part * 3 + 1) * 1024 + tid, ay);
+            xput(xb + (part * 3 + 2) * 1024 + tid, az);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            bool ok = !sync_fail;
+            if (ok) ok = chain_barrier(A.bar + slot, (epoch + 1) * (uint32_t)nparts);
+            sm.ScalI[6] = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (!sm.ScalI[6]) sync_fail = true;
+        ++epoch;
+        if (own) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int g = 0; g < nparts; ++g) {
+                sx += xget(xb + (g * 3 + 0) * 1024 + tid);
+                sy += xget(xb + (g * 3 + 1) * 1024 + tid);
+                sz += xget(xb + (g * 3 + 2) * 1024 + tid);
+            }
+            ax = sx;
+            ay = sy;
+            az = sz;
+        }
+    }
+
     // y = D(sm.R) v   (Diffusion::apply, mobility.cpp:53-56). Each lane runs
     // over all N partners with its own bead included: the self pair evaluates
     // to om1(1e-12) I and cdiag * v restores the identity block (RpyW), so the
@@ -598,8 +682,10 @@ struct Chain {
                 const lds_cptr<double> r0 = as_lds(sm.R[0]), r1 = as_lds(sm.R[1]), r2 = as_lds(sm.R[2]);
                 const lds_cptr<double> v0 = as_lds(sm.X[0]), v1 = as_lds(sm.X[1]), v2 = as_lds(sm.X[2]);
                 const double rx = r0[tid], ry = r1[tid], rz = r2[tid];
+                const int nlo = SPLIT ? part * N / nparts : 0;
+                const int nhi = SPLIT ? (part + 1) * N / nparts : N;
 #pragma unroll 2
-                for (int nu = 0; nu < N; ++nu) {
+                for (int nu = nlo; nu < nhi; ++nu) {
                     const double bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
                     double om1, om2;
                     W.weights(bx * bx + by * by + bz * bz, om1, om2);
@@ -609,9 +695,11 @@ struct Chain {
                     ay = fma(om1, xy, ay); ay = fma(t, by, ay);
                     az = fma(om1, xz, az); az = fma(t, bz, az);
                 }
-                ax = fma(W.cdiag, v[0], ax);
-                ay = fma(W.cdiag, v[1], ay);
-                az = fma(W.cdiag, v[2], az);
+                if constexpr (!SPLIT) {
+                    ax = fma(W.cdiag, v[0], ax);
+                    ay = fma(W.cdiag, v[1], ay);
+                    az = fma(W.cdiag, v[2], az);
+                }
             } else {
                 RpyW<float> W;
                 W.init(P.rpy_f);
@@ -630,8 +718,9 @@ struct Chain {
                     fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);
                     fz = fmaf(om1, xz, fz); fz = fmaf(t, bz, fz);
                 };
-                int nu = 0;
-                for (; nu + 8 <= N; nu += 8) {
+                int nu = SPLIT ? part * N / nparts : 0;
+                const int nhi = SPLIT ? (part + 1) * N / nparts : N;
+                for (; nu + 8 <= nhi; nu += 8) {
                     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
                     for (int k = 0; k < 8; ++k) pair(nu + k, fx, fy, fz);
@@ -639,14 +728,28 @@ struct Chain {
                     ay += (double)fy;
                     az += (double)fz;
                 }
-                if (nu < N) {
+                if (nu < nhi) {
                     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-                    for (; nu < N; ++nu) pair(nu, fx, fy, fz);
+                    for (; nu < nhi; ++nu) pair(nu, fx, fy, fz);
                     ax += (double)fx;
                     ay += (double)fy;
                     az += (double)fz;
                 }
-                const double cd = (double)W.cdiag;
+                if constexpr (!SPLIT) {
+                    const double cd = (double)W.cdiag;
+                    ax = fma(cd, v[0], ax);
+                    ay = fma(cd, v[1], ay);
+                    az = fma(cd, v[2], az);
+                }
+            }
+        }
+        if constexpr (SPLIT) {
+            // every thread of the workgroup takes part in the exchange
+            exchange(ax, ay, az);
+            if (own) {
+                double cd;
+                if constexpr (MIXED) cd = (double)RpyW<float>::cdiag_of(P.rpy_f);
+                else cd = RpyW<double>::cdiag_of(P.rpy_d);
                 ax = fma(cd, v[0], ax);
                 ay = fma(cd, v[1], ay);
                 az = fma(cd, v[2], az);
@@ -1380,10 +1483,14 @@ struct Chain {
                 if (iter > max_iter) bump(sm.cnt.nonconverged);
             }
             bump(sm.cnt.steps);
-            if (logp && step < P.log_cap && tid == 0) logp[step] = lg;
+            if (logp && step < P.log_cap && tid == 0 && part == 0) logp[step] = lg;
             ++step;
+            if constexpr (SPLIT) {
+                if (sync_fail) { status = ST_SYNC_TIMEOUT; break; }
+            }
         }
 
+        if (part != 0) return;      // SPLIT: part 0 writes the results
         if (own)
             for (int d = 0; d < 3; ++d) A.R[P.R_off + 3 * tid + d] = R[d];
         __syncthreads();
@@ -1449,6 +1556,20 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
     init_counters(sm);
     if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
     Chain<BLOCK, MIXED, NB> c(sm, *P, a, dl.blk, dl.xs);
+    c.run(ti);
+}
+
+// Chains of N > 512 split over a.split workgroups each (few chains, many
+// CUs). Launched cooperatively, so that all parts are resident together.
+template <bool MIXED>
+__global__ void __launch_bounds__(1024) k_integrate_split(KArgs a) {
+    __shared__ Smem<1024, MIXED> sm;
+    const int G = a.split;
+    const int ti = a.list[blockIdx.x / G];
+    const TrajParams* __restrict__ P = a.tp + ti;
+    init_counters(sm);
+    Chain<1024, MIXED, 0, true> c(sm, *P, a);
+    c.set_part(blockIdx.x % G, G);
     c.run(ti);
 }
 
@@ -1676,7 +1797,8 @@ struct Ctx {
     hipEvent_t bevent[N_BUCKETS] = {}, uploaded = nullptr;
     uint32_t* jx = nullptr;
     uint32_t* jy = nullptr;
-    DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log, obs;
+    DevBuf tp, list, R, ws, blob, ext, rng, status, cnt, log, obs, bar, xbuf;
+    int coop = 0, n_cu = 256;       // cooperative launch supported; compute units
 } g;
 
 // Columns of T^(3 mu) for the xorshift (13,17,5) map and 16807^(3 mu) mod M.
@@ -1706,6 +1828,9 @@ int ensure_init() {
     if (g.device < 0) g.device = 0;
     HIP_TRY(hipSetDevice(g.device));
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    (void)hipDeviceGetAttribute(&g.coop, hipDeviceAttributeCooperativeLaunch, g.device);
+    (void)hipDeviceGetAttribute(&g.n_cu, hipDeviceAttributeMultiprocessorCount, g.device);
+    if (g.n_cu <= 0) g.n_cu = 256;
     for (int b = 0; b < N_BUCKETS; ++b) {
         HIP_TRY(hipStreamCreateWithFlags(&g.bstream[b], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&g.bevent[b], hipEventDisableTiming));
@@ -1895,6 +2020,41 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     }
 }
 
+// Workgroups per chain for a batch whose only chains are `count` chains of
+// N > 512 (1 = no split): as many as fill the compute units, at most 8.
+// BDSIM_SPLIT=0 in the environment turns the split off.
+int split_factor(int count, int total) {
+    static const bool enabled = [] {
+        const char* e = std::getenv("BDSIM_SPLIT");
+        return !(e && e[0] == '0');
+    }();
+    if (!enabled || !g.coop || count == 0 || count != total) return 1;
+    const int G = std::min(8, g.n_cu / count);
+    return G >= 2 ? G : 1;
+}
+
+// The split launch: count chains x G parts, cooperative (all resident); if
+// the runtime refuses a cooperative launch of that size, the chains run
+// whole, one workgroup each (their first workspace copy).
+template <bool MIXED>
+int launch_split(int count, int G, hipStream_t stream, KArgs a) {
+    if (int rc = g.bar.ensure(sizeof(uint32_t) * count)) return rc;
+    if (int rc = g.xbuf.ensure(sizeof(double) * (size_t)count * 2 * G * 3 * 1024)) return rc;
+    HIP_TRY(hipMemsetAsync(g.bar.p, 0, sizeof(uint32_t) * count, stream));
+    a.split = G;
+    a.bar = static_cast<uint32_t*>(g.bar.p);
+    a.xbuf = static_cast<double*>(g.xbuf.p);
+    void* args[] = {&a};
+    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_integrate_split<MIXED>),
+                                                    dim3(count * G), dim3(1024), args, 0, stream);
+    if (e == hipSuccess) return 0;
+    (void)hipGetLastError();
+    a.split = 1;
+    launch_integrate<MIXED>(1024, dim3(count), stream, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <bool MIXED>
 int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a, int nb, long long waves) {
     if (count == 0) return 0;
@@ -1950,15 +2110,13 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
     std::vector<TrajParams> tps(n_traj);
     std::vector<double> blob;
     std::vector<ExtEntry> ext;
-    int64_t roff = 0, wsoff = 0;
+    int64_t roff = 0;
     for (int i = 0; i < n_traj; ++i) {
         const int N = nbeads[i];
         if (N < 2 || N > MAX_BEADS) return fail(BDSIM_EINVAL, "nbeads out of range [2, max_beads]");
         if (int rc = pack_traj(phys[i], sim[i], N, blob, ext, tps[i])) return rc;
         tps[i].R_off = roff;
-        tps[i].ws_off = wsoff;
         roff += 3 * N;
-        wsoff += ws_layout(N, phys[i].hstar > 0.0).total;
         if (log && log_stride > 0) {
             tps[i].log_off = (int64_t)i * log_stride;
             tps[i].log_cap = log_stride;
@@ -1987,6 +2145,15 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
             if (bucket[i] == b) { order.push_back(i); ++counts[b]; }
         std::stable_sort(order.begin() + first, order.end(),
                          [&](int32_t x, int32_t y) { return nbeads[x] > nbeads[y]; });
+    }
+    // Chains of N > 512 alone in the batch and too few to fill the device:
+    // each is split over G workgroups (one per CU), with G workspaces.
+    const int split = split_factor(counts[4], (int)order.size());
+    int64_t wsoff = 0;
+    for (int i = 0; i < n_traj; ++i) {
+        tps[i].ws_off = wsoff;
+        const int copies = (bucket[i] == 4 && split > 1) ? split : 1;
+        wsoff += copies * ws_layout(nbeads[i], phys[i].hstar > 0.0).total;
     }
 
     if (int rc = g.tp.ensure(sizeof(TrajParams) * n_traj)) return rc;
@@ -2037,9 +2204,15 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
         if (counts[b] == 0) continue;
         HIP_TRY(hipStreamWaitEvent(g.bstream[b], g.uploaded, 0));
         a.list = as<int32_t>(g.list) + starts[b];
-        int rc = precision == BDSIM_PRECISION_MIXED
+        int rc;
+        if (b == 4 && split > 1) {
+            rc = precision == BDSIM_PRECISION_MIXED ? launch_split<true>(counts[b], split, g.bstream[b], a)
+                                                    : launch_split<false>(counts[b], split, g.bstream[b], a);
+        } else {
+            rc = precision == BDSIM_PRECISION_MIXED
                      ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves)
                      : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves);
+        }
         if (rc) {
             (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
             return rc;

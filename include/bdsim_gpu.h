@@ -60,7 +60,11 @@ enum {
     BDSIM_STATUS_RTSAFE_MAXIT = 3,          /* rtsafe abort (numerics.hpp:71-72)     */
     BDSIM_STATUS_EXACT_SQRT = 4,            /* ExactSqrt abort (mobility.cpp:83-85)  */
     BDSIM_STATUS_WLCB_SIGMA = 5,            /* sigma/L > 1 abort (spring.cpp:101-105)*/
-    BDSIM_STATUS_NOT_RUN = 6                /* batch entry was skipped               */
+    BDSIM_STATUS_NOT_RUN = 6,               /* batch entry was skipped               */
+    BDSIM_STATUS_SYNC_TIMEOUT = 7           /* a chain split over several workgroups
+                                               lost a part (they were not resident
+                                               together); no counterpart in the
+                                               reference                            */
 };
 
 /* API error codes (negative return values). */

@@ -271,6 +271,36 @@ def test_long_chains_n1000(gpu, orc):
         assert rel(out[i], want[i]) < 1e-5, (i, rel(out[i], want[i]))
 
 
+@pytest.mark.parametrize("n,N", [(3, 700), (40, 520)])
+def test_split_chains(gpu, orc, n, N):
+    """Batches whose chains all have N > 512 and leave compute units free run
+    each chain on several workgroups (8 for 3 chains, 6 for 40 on 256 CUs),
+    every D.x split by partners: 4 steps of the first chains within 1e-10 of
+    the FP64 oracle with the same decisions, mixed within 1e-5, RNG states as
+    the oracle's."""
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=1, zstar=0.3, dstar=0.5,
+                    kappa=C.wi_kappa(0.05))
+    p = H.Packed(d, N)
+    sim = H.csim(C.sim_dict(0.0, 0.003, 1e-3, 1e-3))
+    R0 = [W.self_avoiding_walk(N, np.random.default_rng(50 + i)) for i in range(n)]
+    items = [(N, R0[i], p.c, sim, H.rng_of(i)) for i in range(n)]
+    out, rngs, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=8)
+    assert np.all(status == 0)
+    mixed, _, status_m, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    assert np.all(status_m == 0)
+    for i in range(2):
+        st = H.rng_of(i)
+        want, rc, cnto, lo = H.oracle_integrate(orc, N, R0[i], p.c, sim, st, log_cap=8)
+        assert rc == 0 and cnts[i].steps == cnto.steps == 4
+        assert rel(out[i], want) <= 1e-10, (i, rel(out[i], want))
+        assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st.ix, st.iy, st.seed)
+        for k in range(4):
+            a, b = logs[i * 8 + k], lo[k]
+            assert (a.lanczos_steps, a.cheb_L, a.cheb_attempts, a.corrector_iters) == \
+                (b.lanczos_steps, b.cheb_L, b.cheb_attempts, b.corrector_iters), (i, k)
+        assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))
+
+
 def test_mixed_precision_within_1e5(gpu, orc):
     """Headline physics (N=50, FENE b=100, h*=0.3 Chebyshev, SDK, shear):
     the mixed form stays within 1e-5 relative of the FP64 oracle over 100 steps."""
