@@ -942,20 +942,23 @@ struct Chain {
                 for (int d = 0; d < 3; ++d) sm.W[d][tid] = (float)(C[d][tid] - C[d][0]);
             __syncthreads();
         }
-        if (!own) return;
-        // The self pair takes the factor's value at Rmin, finite in FP32 for
-        // Rmin >= 0.1; a smaller soft-core radius keeps the explicit exclusion
-        // test, which discards that value.
-        const bool cdt = P.cd > 1 || (et != 1 && !(P.Rmin2 >= 1.0e-2));
-        if (et == 1) {
-            if (cdt) ev_loop<1, false, true>(C, F); else ev_loop<1, false, false>(C, F);
-        } else if (et == 2) {
-            if (cdt) ev_loop<2, false, true>(C, F); else ev_loop<2, false, false>(C, F);
-        } else if (P.has_phi && !P.equil) {
-            ev_loop<3, true, true>(C, F);
-        } else {
-            if (cdt) ev_loop<3, false, true>(C, F); else ev_loop<3, false, false>(C, F);
+        if (own) {
+            // The self pair takes the factor's value at Rmin, finite in FP32
+            // for Rmin >= 0.1; a smaller soft-core radius keeps the explicit
+            // exclusion test, which discards that value.
+            const bool cdt = P.cd > 1 || (et != 1 && !(P.Rmin2 >= 1.0e-2));
+            if (et == 1) {
+                if (cdt) ev_loop<1, false, true>(C, F); else ev_loop<1, false, false>(C, F);
+            } else if (et == 2) {
+                if (cdt) ev_loop<2, false, true>(C, F); else ev_loop<2, false, false>(C, F);
+            } else if (P.has_phi && !P.equil) {
+                ev_loop<3, true, true>(C, F);
+            } else {
+                if (cdt) ev_loop<3, false, true>(C, F); else ev_loop<3, false, false>(C, F);
+            }
         }
+        // SPLIT: each part summed its share of the partners
+        if constexpr (SPLIT) exchange(F[0], F[1], F[2]);
     }
 
     // The pair factor (excluded_volume.cpp:36-58) from the clamped r^2: Gauss
@@ -1010,25 +1013,27 @@ struct Chain {
             fy = fma(-fac, qy, fy);
             fz = fma(-fac, qz, fz);
         };
+        const int nlo = SPLIT ? part * N / nparts : 0;
+        const int nhi = SPLIT ? (part + 1) * N / nparts : N;
         if constexpr (MIXED) {
             double ax = 0.0, ay = 0.0, az = 0.0;
-            int nu = 0;
-            for (; nu + 8 <= N; nu += 8) {
+            int nu = nlo;
+            for (; nu + 8 <= nhi; nu += 8) {
                 float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) pair(nu + k, fx, fy, fz);
                 ax += (double)fx; ay += (double)fy; az += (double)fz;
             }
-            if (nu < N) {
+            if (nu < nhi) {
                 float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-                for (; nu < N; ++nu) pair(nu, fx, fy, fz);
+                for (; nu < nhi; ++nu) pair(nu, fx, fy, fz);
                 ax += (double)fx; ay += (double)fy; az += (double)fz;
             }
             F[0] = ax; F[1] = ay; F[2] = az;
         } else {
             double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll 2
-            for (int nu = 0; nu < N; ++nu) pair(nu, fx, fy, fz);
+            for (int nu = nlo; nu < nhi; ++nu) pair(nu, fx, fy, fz);
             F[0] = fx; F[1] = fy; F[2] = fz;
         }
     }
