@@ -2041,14 +2041,15 @@ int split_factor(int count, int total) {
 // The split launch: count chains x G parts, cooperative (all resident); if
 // the runtime refuses a cooperative launch of that size, the chains run
 // whole, one workgroup each (their first workspace copy).
+// bar: count counters; xbuf: count * split_xbuf_doubles(G) doubles.
+size_t split_xbuf_doubles(int G) { return (size_t)2 * G * 3 * 1024; }
+
 template <bool MIXED>
-int launch_split(int count, int G, hipStream_t stream, KArgs a) {
-    if (int rc = g.bar.ensure(sizeof(uint32_t) * count)) return rc;
-    if (int rc = g.xbuf.ensure(sizeof(double) * (size_t)count * 2 * G * 3 * 1024)) return rc;
-    HIP_TRY(hipMemsetAsync(g.bar.p, 0, sizeof(uint32_t) * count, stream));
+int launch_split(int count, int G, hipStream_t stream, KArgs a, uint32_t* bar, double* xbuf) {
+    HIP_TRY(hipMemsetAsync(bar, 0, sizeof(uint32_t) * count, stream));
     a.split = G;
-    a.bar = static_cast<uint32_t*>(g.bar.p);
-    a.xbuf = static_cast<double*>(g.xbuf.p);
+    a.bar = bar;
+    a.xbuf = xbuf;
     void* args[] = {&a};
     const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_integrate_split<MIXED>),
                                                     dim3(count * G), dim3(1024), args, 0, stream);
@@ -2074,7 +2075,7 @@ T* as(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
 }  // namespace
 
 struct bdsim_batch {
-    int n_traj = 0, N = 0, precision = 0, block = 64, dense_nb = 0;
+    int n_traj = 0, N = 0, precision = 0, block = 64, dense_nb = 0, split = 1;
     hipStream_t stream = nullptr;
     TrajParams* tp = nullptr;
     int32_t* list = nullptr;
@@ -2086,6 +2087,8 @@ struct bdsim_batch {
     int32_t* status = nullptr;
     Counters* cnt = nullptr;
     void* obs = nullptr;            // observables records, allocated on first use
+    uint32_t* bar = nullptr;        // split chains: barrier counters, exchange buffers
+    double* xbuf = nullptr;
 };
 
 extern "C" {
@@ -2211,8 +2214,12 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
         a.list = as<int32_t>(g.list) + starts[b];
         int rc;
         if (b == 4 && split > 1) {
-            rc = precision == BDSIM_PRECISION_MIXED ? launch_split<true>(counts[b], split, g.bstream[b], a)
-                                                    : launch_split<false>(counts[b], split, g.bstream[b], a);
+            if (int e = g.bar.ensure(sizeof(uint32_t) * counts[b])) return e;
+            if (int e = g.xbuf.ensure(sizeof(double) * counts[b] * split_xbuf_doubles(split))) return e;
+            uint32_t* bar = static_cast<uint32_t*>(g.bar.p);
+            double* xb = static_cast<double*>(g.xbuf.p);
+            rc = precision == BDSIM_PRECISION_MIXED ? launch_split<true>(counts[b], split, g.bstream[b], a, bar, xb)
+                                                    : launch_split<false>(counts[b], split, g.bstream[b], a, bar, xb);
         } else {
             rc = precision == BDSIM_PRECISION_MIXED
                      ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves)
@@ -2254,7 +2261,9 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
     TrajParams proto;
     if (int rc = pack_traj(*phys, s0, N, blob, ext, proto)) return rc;
     std::vector<TrajParams> tps(n_traj, proto);
-    const int64_t wsper = ws_layout(N, phys->hstar > 0.0).total;
+    // chains of N > 512 too few to fill the device run split (G workspaces)
+    const int split = block_for(N) == 1024 ? split_factor(n_traj, n_traj) : 1;
+    const int64_t wsper = split * ws_layout(N, phys->hstar > 0.0).total;
     std::vector<int32_t> list(n_traj);
     for (int i = 0; i < n_traj; ++i) {
         tps[i].R_off = (int64_t)i * 3 * N;
@@ -2274,6 +2283,7 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
     b->precision = precision;
     b->block = block_for(N);
     b->dense_nb = dense_nb_for(N, phys->hstar, phys->hi_method);
+    b->split = split;
     auto cleanup = [&](int code, const char* what) {
         bdsim_batch_destroy(b);
         return fail(code, what);
@@ -2289,6 +2299,10 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
         hipMalloc(&b->rng, sizeof(RngState) * n_traj) != hipSuccess ||
         hipMalloc(&b->status, sizeof(int32_t) * n_traj) != hipSuccess ||
         hipMalloc(&b->cnt, sizeof(Counters) * n_traj) != hipSuccess)
+        return cleanup(BDSIM_ENOMEM, "hipMalloc failed for batch");
+    if (split > 1 &&
+        (hipMalloc(&b->bar, sizeof(uint32_t) * n_traj) != hipSuccess ||
+         hipMalloc(&b->xbuf, sizeof(double) * n_traj * split_xbuf_doubles(split)) != hipSuccess))
         return cleanup(BDSIM_ENOMEM, "hipMalloc failed for batch");
     if (hipMemcpy(b->tp, tps.data(), sizeof(TrajParams) * n_traj, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->list, list.data(), sizeof(int32_t) * n_traj, hipMemcpyHostToDevice) != hipSuccess ||
@@ -2322,6 +2336,10 @@ int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
     a.t1 = sim->time_end;
     a.dt = sim->dt;
     a.tol = sim->implicit_loop_tol;
+    if (b->split > 1)
+        return b->precision == BDSIM_PRECISION_MIXED
+                   ? launch_split<true>(b->n_traj, b->split, b->stream, a, b->bar, b->xbuf)
+                   : launch_split<false>(b->n_traj, b->split, b->stream, a, b->bar, b->xbuf);
     if (b->precision == BDSIM_PRECISION_MIXED) launch_integrate<true>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb);
     else launch_integrate<false>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb);
     HIP_TRY(hipGetLastError());
@@ -2353,6 +2371,7 @@ void bdsim_batch_destroy(bdsim_batch* b) {
     (void)hipFree(b->tp); (void)hipFree(b->list); (void)hipFree(b->R); (void)hipFree(b->ws);
     (void)hipFree(b->blob); (void)hipFree(b->ext); (void)hipFree(b->rng);
     (void)hipFree(b->status); (void)hipFree(b->cnt); (void)hipFree(b->obs);
+    (void)hipFree(b->bar); (void)hipFree(b->xbuf);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

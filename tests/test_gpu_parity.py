@@ -301,6 +301,40 @@ def test_split_chains(gpu, orc, n, N):
         assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))
 
 
+def test_resident_split_chains_match(gpu):
+    """A resident batch of 3 chains of N=600 (split over 8 workgroups each)
+    over two segments equals bdsim_integrate_batch on the same chains (split
+    the same way) bit for bit."""
+    n, N = 3, 600
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=2, zstar=0.5, dstar=1.0, max_cutoff=2.5)
+    p = H.Packed(d, N)
+    R0 = np.ascontiguousarray(np.array([W.self_avoiding_walk(N, np.random.default_rng(70 + i))
+                                        for i in range(n)]))
+    rng = (A.CRng * n)(*[H.rng_of(i) for i in range(n)])
+    b = ctypes.c_void_p()
+    A.check(gpu, gpu.bdsim_batch_create(byref(b), n, N, A.dptr(R0), byref(p.c), None, rng,
+                                        A.PRECISION_MIXED))
+    sim = A.CSim(0.0, 0.002, 1e-3, 1e-3, 1)
+    try:
+        A.check(gpu, gpu.bdsim_batch_run(b, byref(sim)))
+        A.check(gpu, gpu.bdsim_batch_run(b, byref(sim)))
+        Rb = np.empty_like(R0)
+        st = (A.CRng * n)()
+        status = np.zeros(n, dtype=np.int32)
+        A.check(gpu, gpu.bdsim_batch_fetch(b, A.dptr(Rb), st, A.iptr(status), None))
+    finally:
+        gpu.bdsim_batch_destroy(b)
+    assert np.all(status == 0)
+    items = [(N, R0[i], p.c, sim, H.rng_of(i)) for i in range(n)]
+    for _ in range(2):
+        out, rngs, stt, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+        assert np.all(stt == 0)
+        items = [(N, out[i], p.c, sim, rngs[i]) for i in range(n)]
+    for i in range(n):
+        assert np.array_equal(out[i], Rb[i]), i
+        assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st[i].ix, st[i].iy, st[i].seed)
+
+
 def test_mixed_precision_within_1e5(gpu, orc):
     """Headline physics (N=50, FENE b=100, h*=0.3 Chebyshev, SDK, shear):
     the mixed form stays within 1e-5 relative of the FP64 oracle over 100 steps."""
