@@ -11,6 +11,9 @@ uneven: 4096 chains with N log-uniform in [10, 256], the headline physics.
 cholesky: 8192 chains of N=30 with the dense Cholesky square root (a 90 x 90
 factor per chain and step), Gaussian excluded volume, planar extension.
 
+c1: the reference's CPU-sized case, 16 chains of N=10 in 10 segments of 100
+steps with snapshot observables after each.
+
 long: 64 chains of N=1000 with bending and Lennard-Jones excluded volume,
 self-avoiding starts.
 """
@@ -131,3 +134,22 @@ def cholesky_workload(n_traj=8192, N=30, seed=4):
     c.hstar, c.hi_method = 0.25, 1
     c.flow.kappa0[0], c.flow.kappa0[4] = 0.5, -0.5
     return np.full(n_traj, N, dtype=np.int32), R, [c] * n_traj, 1e-3
+
+
+def c1_workload(n_traj=16, N=10):
+    """16 trajectories (seeds 0..15) of N=10, FENE-Fraenkel springs sqrtb=2
+    natural length 1 started from the FENE-Fraenkel sampler (sigma=1, dQ=2),
+    RPY h*=0.3 Chebyshev, no EV, simple shear rate 1, dt=0.01; 1000 steps as
+    10 segments of 100 (write_every=100). Returns (R0 (n, N, 3), PhysParams,
+    SimParams, segments, steps per segment)."""
+    from .core import DelSMethod, Flow, PhysParams, SimParams, Spring
+    from .initial import fene_fraenkel_chain
+    p = PhysParams()
+    p.number_of_beads = N
+    p.spring.type, p.spring.sqrtb, p.spring.natural_length = Spring.FENEFraenkel, 2.0, 1.0
+    p.hstar, p.hi_method = 0.3, DelSMethod.Chebyshev
+    p.flow = Flow.constant([[0.0, 1.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    s = SimParams()
+    s.time_start, s.time_end, s.dt, s.implicit_loop_tol = 0.0, 9.99, 0.01, 1e-3
+    R0 = np.array([fene_fraenkel_chain(N, 1.0, 2.0, seed=i) for i in range(n_traj)])
+    return R0, p, s, 10, 100

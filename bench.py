@@ -259,6 +259,60 @@ def cpu_baseline(n_traj_sample=768, steps=400):
                           "; FP64 C restatement (reference build absent)"))}
 
 
+def run_c1(precision):
+    """The CPU-sized case on the GPU: 16 chains resident, 10 segments of 100
+    steps, Rg^2 / Q^2 / tau_xy evaluated on the device after each segment; the
+    time covers the whole run including the observables' copies. Beside it the
+    same 16 x 10 segments integrated on one host core by the reference build
+    (or the FP64 restatement)."""
+    from bdsim_amd.resident import ResidentEnsemble
+    from bdsim_amd.workloads import c1_workload
+    R0, phys, sim, nseg, per = c1_workload()
+    n, N = R0.shape[0], R0.shape[1]
+    t = time.perf_counter()
+    with ResidentEnsemble(R0, phys, sim, list(range(n)), precision) as ens:
+        series = []
+        for _ in range(nseg):
+            ens.run(per)
+            o = ens.observe(force="total")
+            series.append((o["rg2"].mean(), o["q2"].mean(), o["stress"][:, 0, 1].mean()))
+        _, _, status, _ = ens.fetch()
+    wall = time.perf_counter() - t
+    out = {"bead_steps_per_s": n * N * nseg * per / wall, "wall_s": wall, "n_traj": n,
+           "status_ok": int(np.sum(status == 0)),
+           "last_snapshot": {"rg2": float(series[-1][0]), "q2": float(series[-1][1]),
+                             "tau_xy": float(series[-1][2])}}
+    return out
+
+
+def c1_cpu():
+    """16 x 10 segments of 100 steps on one host core (reference build, else
+    the FP64 restatement), from the same starts and seeds."""
+    from bdsim_amd.workloads import c1_workload
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import helpers as H
+    kind = "reference" if os.path.exists(H.REF_PATH) else "port"
+    lib = H.ref() if kind == "reference" else H.oracle()
+    R0, phys, sim, nseg, per = c1_workload()
+    pk = A.PackedPhys(phys, R0.shape[1])
+    n, N = R0.shape[0], R0.shape[1]
+    t = time.perf_counter()
+    for i in range(n):
+        R = np.ascontiguousarray(R0[i]).copy()
+        st = A.CRng(1, -1, i)
+        for k in range(nseg):
+            t0 = k * per * sim.dt
+            s = A.CSim(t0, t0 + (per - 1) * sim.dt, sim.dt, sim.implicit_loop_tol, 1)
+            if kind == "reference":
+                lib.ref_integrate(N, A.dptr(R), byref(pk.c), byref(s), byref(st))
+            else:
+                cnt = A.CCounters()
+                lib.bo_integrate(N, A.dptr(R), byref(pk.c), byref(s), byref(st), byref(cnt), None, 0)
+    wall = time.perf_counter() - t
+    return {"value": n * N * nseg * per / wall, "unit": "bead-steps/s", "cores": 1, "kind": kind,
+            "sample": "the whole case: 16 chains x 10 segments of 100 steps, one process"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,11 +326,12 @@ def main():
     ap.add_argument("--serving-step-scale", type=float, default=0.02)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
-    ap.add_argument("--workload", choices=["headline", "uneven", "long", "cholesky"],
+    ap.add_argument("--workload", choices=["headline", "uneven", "long", "cholesky", "c1"],
                     default="headline",
                     help="uneven: 4096 chains with N log-uniform in [10,256]; "
                          "long: 64 chains of N=1000; cholesky: 8192 chains of N=30 with the "
-                         "dense Cholesky square root (each prints its own line)")
+                         "dense Cholesky square root; c1: 16 chains of N=10 in 10 segments "
+                         "with snapshot observables (each prints its own line)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -290,7 +345,15 @@ def main():
         for form in args.forms.split(","):
             prec = A.PRECISION_MIXED if form == "mixed" else A.PRECISION_FP64
             print(f"bench: {args.workload} {form}", file=sys.stderr, flush=True)
-            out[form] = run_workload(lib, args.workload, prec, args.steps, args.warmup)
+            if args.workload == "c1":
+                out[form] = run_c1(prec)
+            else:
+                out[form] = run_workload(lib, args.workload, prec, args.steps, args.warmup)
+        if args.workload == "c1":
+            out["steps"], out["warmup"] = 1000, 0
+            if not args.no_cpu:
+                print("bench: c1 cpu", file=sys.stderr, flush=True)
+                out["cpu_baseline"] = c1_cpu()
         print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:

@@ -39,3 +39,13 @@ def test_long_workload_self_avoiding():
     p = phys[0]
     assert (p.ev_type, p.zstar, p.dstar, p.max_cutoff) == (2, 0.5, 1.0, 2.5)
     assert (p.bend_type, p.bend_stiffness, p.hstar) == (1, 5.0, 0.2)
+
+
+def test_c1_workload():
+    R0, phys, sim, nseg, per = W.c1_workload()
+    assert R0.shape == (16, 10, 3) and nseg * per == 1000 and sim.dt == 0.01
+    # FENE-Fraenkel bonds in [sigma - dQ, sigma + dQ] = [0, 3]
+    q = np.linalg.norm(np.diff(R0, axis=1), axis=2)
+    assert q.max() <= 3.0 and q.min() >= 0.0
+    assert (int(phys.spring.type), phys.spring.sqrtb, phys.spring.natural_length) == (6, 2.0, 1.0)
+    assert phys.flow.constant_kappa()[0][1] == 1.0 and phys.hstar == 0.3
