@@ -200,6 +200,8 @@ def _declare_gpu(lib):
     lib.bdsim_init.argtypes = [c_int]
     lib.bdsim_last_error.restype = ctypes.c_char_p
     lib.bdsim_max_beads.restype = c_int
+    lib.bdsim_tune_set.argtypes = [c_int, c_int]
+    lib.bdsim_tune_get.argtypes = [c_int]
     lib.bdsim_integrate_batch.argtypes = [c_int, _ip, _dp, P(CPhys), P(CSim), P(CRng), c_int,
                                           _ip, P(CCounters), P(CStepLog), c_int]
     lib.bdsim_batch_create.argtypes = [P(c_void_p), c_int, c_int, _dp, P(CPhys), _dp,

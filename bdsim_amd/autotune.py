@@ -1,0 +1,99 @@
+"""Per-bucket register budgets of the integrate kernels, picked by timing.
+
+The library launches the chains of a batch in buckets by block size (64, 128,
+256, 512 threads for chains of up to that many beads). Each bucket's kernel
+exists in builds for different register budgets (waves per SIMD: 1, 2 or 4 for
+64-thread chains; 1 or 4 for the others); more registers per wave means fewer
+spills, fewer means more chains resident at once. Which wins depends on the
+mix, so `tune` times the candidates on a representative batch -- one bucket at
+a time, the others on the library's own choice, all buckets running side by
+side as they do in use -- and keeps the fastest. The table applies with
+`apply` (bdsim_tune_set) and round-trips through JSON (`save` / `load`).
+Results do not depend on the budget (the same operations, another register
+allocation), so tuning changes time only.
+"""
+import json
+import time
+
+import numpy as np
+
+from . import _abi
+
+CANDIDATES = {64: (1, 2, 4), 128: (1, 4), 256: (1, 4), 512: (1, 4)}
+
+
+def block_of(N):
+    """The library's block size for a chain of N beads: 64, 128, 256, 512 or 1024."""
+    b = 64
+    while b < int(N):
+        b *= 2
+    return b
+
+
+def apply(table, lib=None):
+    """Set the library's budgets from {block: waves} (0 = automatic)."""
+    lib = lib or _abi.load_gpu()
+    for block, w in table.items():
+        _abi.check(lib, lib.bdsim_tune_set(int(block), int(w)))
+
+
+def reset(lib=None):
+    apply({b: 0 for b in CANDIDATES}, lib)
+
+
+def current(lib=None):
+    lib = lib or _abi.load_gpu()
+    return {b: int(lib.bdsim_tune_get(b)) for b in CANDIDATES}
+
+
+def save(table, path):
+    with open(path, "w") as f:
+        json.dump({str(k): int(v) for k, v in table.items()}, f, indent=1, sort_keys=True)
+
+
+def load(path):
+    with open(path) as f:
+        return {int(k): int(v) for k, v in json.load(f).items()}
+
+
+def _timed(lib, N, R, phys, dt, precision, steps, warmup):
+    n = len(N)
+    Rf = np.ascontiguousarray(np.concatenate([np.asarray(r, dtype=np.float64).ravel() for r in R]))
+    ph = (_abi.CPhys * n)(*phys)
+    rng = (_abi.CRng * n)(*[_abi.CRng(1, -1, i) for i in range(n)])
+    st = np.zeros(n, dtype=np.int32)
+    Nc = np.ascontiguousarray(np.asarray(N, dtype=np.int32))
+
+    def segment(t0, k):
+        sims = (_abi.CSim * n)(*[_abi.CSim(t0, t0 + (k - 1) * dt, dt, 1e-3, 1)] * n)
+        t = time.perf_counter()
+        _abi.check(lib, lib.bdsim_integrate_batch(n, _abi.iptr(Nc), _abi.dptr(Rf), ph, sims, rng,
+                                                  precision, _abi.iptr(st), None, None, 0))
+        return time.perf_counter() - t
+
+    if warmup > 0:
+        segment(0.0, warmup)
+    return segment(warmup * dt, steps)
+
+
+def tune(N, R, phys, dt, precision=_abi.PRECISION_MIXED, steps=20, warmup=3, repeats=2,
+         lib=None, verbose=False):
+    """Fastest budget per bucket present in the batch (N: chain lengths, R:
+    their positions, phys: CPhys per chain, dt). Returns {block: waves}; the
+    library is left on that table."""
+    lib = lib or _abi.load_gpu()
+    blocks = sorted({block_of(n) for n in N if block_of(n) in CANDIDATES})
+    table = {}
+    reset(lib)
+    for b in blocks:
+        best = None
+        for w in CANDIDATES[b]:
+            apply({b: w}, lib)
+            t = min(_timed(lib, N, R, phys, dt, precision, steps, warmup) for _ in range(repeats))
+            if verbose:
+                print(f"autotune: block {b} waves/SIMD {w}: {t * 1e3 / steps:.3f} ms/step")
+            if best is None or t < best[1]:
+                best = (w, t)
+        table[b] = best[0]
+        apply({b: best[0]}, lib)
+    return table

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -1949,6 +1950,19 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
 // Register budget of the integrate kernel for 64-thread workgroups: waves per
 // SIMD the allocation is held to (BDSIM_WPE overrides; 1, 2 or 4; default 4,
 // the fastest measured at 4096 x N=50).
+// Register budget per block size set by the caller (bdsim_tune_set): waves
+// per SIMD, 0 = the library's own choice. Index: 64, 128, 256, 512.
+std::atomic<int> g_tune[4] = {{0}, {0}, {0}, {0}};
+int tune_index(int block) {
+    switch (block) {
+        case 64: return 0;
+        case 128: return 1;
+        case 256: return 2;
+        case 512: return 3;
+        default: return -1;
+    }
+}
+
 // More waves than 3 per SIMD of the device: the launch would not be resident
 // at once with the unconstrained register allocation.
 bool crowded(long long waves) {
@@ -1998,7 +2012,11 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
         hipLaunchKernelGGL((k_integrate_dense<64, MIXED, 6>), grid, dim3(64), 0, stream, a);
         return;
     }
-    const int w = wpe_choice();
+    const int ti = tune_index(block);
+    const int tuned = ti >= 0 ? g_tune[ti].load() : 0;
+    // multi-wave blocks: the 4-waves build if tuned so, or by the crowding test
+    const bool four = tuned ? tuned == 4 : crowded(waves);
+    const int w = tuned ? tuned : wpe_choice();
     switch (block) {
         case 64:
             if (w == 4) hipLaunchKernelGGL((k_integrate<64, MIXED, 4>), grid, dim3(64), 0, stream, a);
@@ -2010,15 +2028,15 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
         // stay resident; measured 2x on 4096 chains of N in [10, 256]),
         // otherwise the unconstrained build.
         case 128:
-            if (crowded(waves)) hipLaunchKernelGGL((k_integrate<128, MIXED, 4>), grid, dim3(128), 0, stream, a);
+            if (four) hipLaunchKernelGGL((k_integrate<128, MIXED, 4>), grid, dim3(128), 0, stream, a);
             else hipLaunchKernelGGL((k_integrate<128, MIXED, 1>), grid, dim3(128), 0, stream, a);
             break;
         case 256:
-            if (crowded(waves)) hipLaunchKernelGGL((k_integrate<256, MIXED, 4>), grid, dim3(256), 0, stream, a);
+            if (four) hipLaunchKernelGGL((k_integrate<256, MIXED, 4>), grid, dim3(256), 0, stream, a);
             else hipLaunchKernelGGL((k_integrate<256, MIXED, 1>), grid, dim3(256), 0, stream, a);
             break;
         case 512:
-            if (crowded(waves)) hipLaunchKernelGGL((k_integrate<512, MIXED, 4>), grid, dim3(512), 0, stream, a);
+            if (four) hipLaunchKernelGGL((k_integrate<512, MIXED, 4>), grid, dim3(512), 0, stream, a);
             else hipLaunchKernelGGL((k_integrate<512, MIXED, 1>), grid, dim3(512), 0, stream, a);
             break;
         default: hipLaunchKernelGGL((k_integrate<1024, MIXED, 1>), grid, dim3(1024), 0, stream, a); break;
@@ -2103,6 +2121,21 @@ int bdsim_init(int device) {
 const char* bdsim_last_error(void) { return g_err.c_str(); }
 
 int bdsim_max_beads(void) { return MAX_BEADS; }
+
+int bdsim_tune_set(int block, int waves_per_simd) {
+    const int i = tune_index(block);
+    if (i < 0) return fail(BDSIM_EINVAL, "tune: block must be 64, 128, 256 or 512");
+    const bool ok = waves_per_simd == 0 || waves_per_simd == 1 || waves_per_simd == 4 ||
+                    (block == 64 && waves_per_simd == 2);
+    if (!ok) return fail(BDSIM_EINVAL, "tune: waves per SIMD must be 0, 1 or 4 (or 2 for block 64)");
+    g_tune[i].store(waves_per_simd);
+    return 0;
+}
+
+int bdsim_tune_get(int block) {
+    const int i = tune_index(block);
+    return i < 0 ? -1 : g_tune[i].load();
+}
 
 int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bdsim_phys* phys,
                           const bdsim_sim* sim, bdsim_rng_state* rng, int precision,

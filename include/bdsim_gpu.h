@@ -173,6 +173,15 @@ const char* bdsim_last_error(void);
 /* Largest supported chain length (beads per trajectory). */
 int bdsim_max_beads(void);
 
+/* Register budget of the integrate kernels per block size (64, 128, 256 or
+ * 512 threads, i.e. chains of up to that many beads): waves per SIMD, 1 or
+ * 4 (and 2 for 64); 0 restores the library's own choice (4 for 64; for the
+ * others 4 when a launch holds more waves than 3 per SIMD, else 1). Results do
+ * not depend on it. Set by bdsim_amd.autotune from timed runs; no counterpart
+ * in the reference.                                                          */
+int bdsim_tune_set(int block, int waves_per_simd);
+int bdsim_tune_get(int block);           /* current setting, -1 for a bad block */
+
 /* ---- the step: batched time_integrate_chain ---------------------------- */
 
 /* Integrates n_traj independent chains, each over its own segment
