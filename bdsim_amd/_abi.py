@@ -219,6 +219,7 @@ def _declare_gpu(lib):
     lib.bdsim_external_force_batch.argtypes = [P(CPhys), c_int, c_double, _dp]
     lib.bdsim_raw_noise_batch.argtypes = [c_int, c_int, c_int, P(CRng), _dp]
     lib.bdsim_rpy_apply_batch.argtypes = [c_int, c_int, c_double, c_int, _dp, _dp, _dp]
+    lib.bdsim_brownian_batch.argtypes = [c_int, c_int, _dp, P(CPhys), c_int, _dp, _dp, _ip]
     lib.bdsim_server_create.argtypes = [P(c_void_p), c_int, ctypes.c_int64, c_int]
     lib.bdsim_server_submit.argtypes = [c_void_p, ctypes.c_int64, c_int, c_int, _dp, P(CPhys),
                                         P(CSim), P(CRng)]

@@ -1299,6 +1299,26 @@ struct Chain {
         return true;
     }
 
+    // B x0 of the chain's method at the staged configuration with an empty
+    // spectral cache (mobility.cpp:58-88 without the sqrt(dt) factor): x0 for
+    // free draining, L x0 for Cholesky, the Chebyshev series of sqrt(D) x0 on
+    // cold Lanczos bounds. For parity checks of the forms.
+    __device__ int brownian(const double x0[3], double bx0[3]) {
+        if (!hi) {
+            for (int d = 0; d < 3; ++d) bx0[d] = x0[d];
+            return ST_OK;
+        }
+        if (P.hi_method == 1) {
+            if (!dense_factor()) return ST_NOT_PD;
+            lower_matvec(x0, bx0);
+            return ST_OK;
+        }
+        if (P.hi_method != 0) return ST_EXACT_SQRT;
+        save(SLOT_X0, x0);
+        bool seed_valid = false;
+        return chebyshev_bx0(seed_valid, bx0) ? ST_OK : ST_NOT_PD;
+    }
+
     // inlined into each kernel so that the kernel's waves-per-EU budget applies
     __device__ __forceinline__ void run(int ti) {
         double R[3] = {0.0, 0.0, 0.0};
@@ -1654,6 +1674,30 @@ __global__ void __launch_bounds__(BLOCK) k_rpy_apply(KArgs a, const double* x_in
     c.apply(x, y);
     if (tid < N)
         for (int d = 0; d < 3; ++d) out[off + 3 * tid + d] = y[d];
+}
+
+// B x for many chains of one physics (positions and x back to back), each
+// with its own workspace.
+template <int BLOCK, bool MIXED>
+__global__ void __launch_bounds__(BLOCK) k_brownian(KArgs a, const double* x_in, double* out,
+                                                    int32_t* status) {
+    __shared__ Smem<BLOCK, MIXED> sm;
+    const int ti = blockIdx.x;
+    const TrajParams* __restrict__ P = a.tp;
+    init_counters(sm);
+    Chain<BLOCK, MIXED> c(sm, *P, a);
+    c.ws = as_gbl(a.ws + (int64_t)ti * c.wl.total);
+    const int N = P->N, tid = threadIdx.x;
+    const int64_t off = (int64_t)ti * 3 * N;
+    double R[3] = {0.0, 0.0, 0.0}, x[3] = {0.0, 0.0, 0.0};
+    if (tid < N)
+        for (int d = 0; d < 3; ++d) { R[d] = a.R[off + 3 * tid + d]; x[d] = x_in[off + 3 * tid + d]; }
+    c.stage_R(R);
+    double y[3];
+    const int st = c.brownian(x, y);
+    if (tid < N)
+        for (int d = 0; d < 3; ++d) out[off + 3 * tid + d] = y[d];
+    if (tid == 0) status[ti] = st;
 }
 
 // ---- snapshot observables (python/bdsim/properties.py:5-52) -----------------
@@ -2480,6 +2524,48 @@ static int launch_observe(int block, int n_traj, hipStream_t stream, const KArgs
 }
 
 static_assert(sizeof(Obs) == sizeof(bdsim_observables), "observables record layout");
+
+}  // extern "C"
+
+template <bool MIXED>
+void launch_brownian(int block, int n, hipStream_t st, const KArgs& a, const double* x, double* y,
+                     int32_t* status) {
+    const dim3 grid(n);
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_brownian<64, MIXED>), grid, dim3(64), 0, st, a, x, y, status); break;
+        case 128: hipLaunchKernelGGL((k_brownian<128, MIXED>), grid, dim3(128), 0, st, a, x, y, status); break;
+        case 256: hipLaunchKernelGGL((k_brownian<256, MIXED>), grid, dim3(256), 0, st, a, x, y, status); break;
+        case 512: hipLaunchKernelGGL((k_brownian<512, MIXED>), grid, dim3(512), 0, st, a, x, y, status); break;
+        default: hipLaunchKernelGGL((k_brownian<1024, MIXED>), grid, dim3(1024), 0, st, a, x, y, status); break;
+    }
+}
+
+extern "C" {
+
+int bdsim_brownian_batch(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
+                         int precision, const double* x, double* y, int32_t* status) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!x || !y || !status || (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED))
+        return fail(BDSIM_EINVAL, "bad argument");
+    KArgs a;
+    if (int rc = upload_shared(n_traj, nbeads, R, phys, a)) return rc;
+    const size_t nd = (size_t)n_traj * 3 * nbeads;
+    const int64_t wsper = ws_layout(nbeads, phys->hstar > 0.0).total;
+    if (int rc = g.ws.ensure(sizeof(double) * std::max<size_t>(1, (size_t)wsper * n_traj))) return rc;
+    if (int rc = g.obs.ensure(sizeof(double) * 2 * nd + sizeof(int32_t) * n_traj)) return rc;
+    a.ws = as<double>(g.ws);
+    double* dx = as<double>(g.obs);
+    double* dy = dx + nd;
+    int32_t* dst = reinterpret_cast<int32_t*>(dy + nd);
+    HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * nd, hipMemcpyHostToDevice, g.stream));
+    if (precision == BDSIM_PRECISION_MIXED) launch_brownian<true>(block_for(nbeads), n_traj, g.stream, a, dx, dy, dst);
+    else launch_brownian<false>(block_for(nbeads), n_traj, g.stream, a, dx, dy, dst);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(y, dy, sizeof(double) * nd, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(status, dst, sizeof(int32_t) * n_traj, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return 0;
+}
 
 int bdsim_observables_batch(int n_traj, int nbeads, const double* R, const bdsim_phys* phys,
                             int force_kind, bdsim_observables* out) {

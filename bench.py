@@ -313,6 +313,67 @@ def c1_cpu():
             "sample": "the whole case: 16 chains x 10 segments of 100 steps, one process"}
 
 
+def _batch_positions(lib, precision, R0, K, steps):
+    """Positions of the headline chains after `steps` steps of one form."""
+    phys = headline_phys()
+    n_traj = R0.shape[0]
+    rng = (A.CRng * n_traj)(*[A.CRng(1, -1, i) for i in range(n_traj)])
+    b = ctypes.c_void_p()
+    A.check(lib, lib.bdsim_batch_create(byref(b), n_traj, N_BEADS, A.dptr(R0), byref(phys),
+                                        A.dptr(K), rng, precision))
+    try:
+        sim = A.CSim(0.0, (steps - 1) * 1e-3, 1e-3, 1e-3, 1)
+        A.check(lib, lib.bdsim_batch_run(b, byref(sim)))
+        A.check(lib, lib.bdsim_batch_sync(b))
+        status = np.zeros(n_traj, dtype=np.int32)
+        R = np.empty_like(R0)
+        A.check(lib, lib.bdsim_batch_fetch(b, A.dptr(R), None, A.iptr(status), None))
+    finally:
+        lib.bdsim_batch_destroy(b)
+    assert np.all(status == 0)
+    return R
+
+
+def mixed_errors(lib, n_traj=256, steps=100):
+    """Relative errors of the mixed form against the FP64 form (which matches
+    the FP64 CPU restatement to 1e-10 over 100 steps) on the first n_traj
+    headline chains: positions after `steps` steps from the same start and
+    generator streams, and, at the FP64 positions reached, one step's D.F
+    (F the total force) and B.dW (dW standard normal, the Chebyshev series on
+    cold bounds). Per chain |x_mixed - x_fp64| / |x_fp64|; max and median."""
+    R0 = start_configs(n_traj, N_BEADS)
+    K = kappas(n_traj)
+    Rf = _batch_positions(lib, A.PRECISION_FP64, R0, K, steps)
+    Rm = _batch_positions(lib, A.PRECISION_MIXED, R0, K, steps)
+    phys = headline_phys()
+    R = np.ascontiguousarray(Rf)
+    F = np.empty_like(R)
+    A.check(lib, lib.bdsim_total_force_batch(n_traj, N_BEADS, A.dptr(R), byref(phys), A.dptr(F)))
+    dW = np.ascontiguousarray(np.random.default_rng(7).normal(size=R.shape))
+    out = {}
+    for form, prec in (("fp64", A.PRECISION_FP64), ("mixed", A.PRECISION_MIXED)):
+        DF = np.empty_like(R)
+        A.check(lib, lib.bdsim_rpy_apply_batch(n_traj, N_BEADS, phys.hstar, prec, A.dptr(R),
+                                               A.dptr(F), A.dptr(DF)))
+        BW = np.empty_like(R)
+        st = np.zeros(n_traj, dtype=np.int32)
+        A.check(lib, lib.bdsim_brownian_batch(n_traj, N_BEADS, A.dptr(R), byref(phys), prec,
+                                              A.dptr(dW), A.dptr(BW), A.iptr(st)))
+        assert np.all(st == 0)
+        out[form] = (DF, BW)
+
+    def summary(a, b):
+        e = np.linalg.norm((a - b).reshape(n_traj, -1), axis=1) / \
+            np.linalg.norm(b.reshape(n_traj, -1), axis=1)
+        return {"max": float(e.max()), "median": float(np.median(e))}
+
+    return {"sample": f"first {n_traj} headline chains; positions after {steps} steps",
+            "against": "fp64 form",
+            "D.F": summary(out["mixed"][0], out["fp64"][0]),
+            "B.dW": summary(out["mixed"][1], out["fp64"][1]),
+            "positions": summary(Rm, Rf)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,6 +473,9 @@ def main():
                        "achieved_tflops": f["flops"] / (f["kernel_ms"] / 1e3) / 1e12,
                        "frac_of_78.6": f["flops"] / (f["kernel_ms"] / 1e3) / 1e12 / 78.6}
         out["mixed_over_fp64"] = out["value"] / out["fp64"]["value"]
+    if "fp64" in res and "mixed" in res:
+        print("bench: mixed-form errors", file=sys.stderr, flush=True)
+        out["mixed_errors"] = mixed_errors(lib)
     out["counters"] = {k: {kk: v[kk] for kk in ("matvecs_per_step", "corrector_per_step",
                                                  "lanczos_cold", "fallbacks", "status_ok",
                                                  "finite", "wall_s")} for k, v in res.items()}

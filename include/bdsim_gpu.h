@@ -294,6 +294,14 @@ int bdsim_raw_noise_batch(int n_traj, int nbeads, int n_steps,
 /* y = D(R).x for each chain (RPY with hstar; identity when hstar <= 0). */
 int bdsim_rpy_apply_batch(int n_traj, int nbeads, double hstar, int precision,
                           const double* R, const double* x, double* y);
+/* y = B(R).x for each chain: the Brownian operator of phys's method
+ * (Diffusion::brownian_step, mobility.cpp:58-88, without the sqrt(dt)
+ * factor): x when hstar <= 0, L x for Cholesky, the Chebyshev series of
+ * sqrt(D) x on cold Lanczos bounds (hydrodynamics.cpp:317-337). One physics
+ * for all chains; status [n_traj] per chain. For checks of the forms.        */
+int bdsim_brownian_batch(int n_traj, int nbeads, const double* R,
+                         const bdsim_phys* phys, int precision,
+                         const double* x, double* y, int32_t* status);
 
 /* ---- snapshot observables ----------------------------------------------
  * One record per chain, evaluated on the device where the positions are:
