@@ -519,14 +519,16 @@ __device__ __noinline__ D3 lower_matvec_lds(lds_ptr<double> Lb, lds_ptr<double> 
 // Barrier of the parts of one split chain (thread 0 of each part): count in,
 // then wait until the count reaches `target`. Bounded: false after about a
 // second, so that a part that never arrives cannot hold the others forever.
+// Everything the parts exchange goes through agent-scope atomic stores and
+// loads, which are coherent across the XCDs by themselves; the caller has
+// waited for its workgroup's stores to complete (s_waitcnt, then the
+// workgroup barrier), so the arrival needs no release fence, and the reads
+// that follow need no acquire (a fence at agent scope writes back and
+// invalidates the whole L2 of the XCD, every workgroup's cached data).
 __device__ __noinline__ bool chain_barrier(uint32_t* bar, uint32_t target) {
-    __threadfence();
     __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int it = 0; it < (1 << 20); ++it) {
-        if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
-            __threadfence();
-            return true;
-        }
+        if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
         __builtin_amdgcn_s_sleep(1);
     }
     return false;
@@ -631,6 +633,7 @@ struct Chain {
             xput(xb + (part * 3 + 1) * 1024 + tid, ay);
             xput(xb + (part * 3 + 2) * 1024 + tid, az);
         }
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's partials have completed
         __syncthreads();
         if (tid == 0) {
             bool ok = !sync_fail;
