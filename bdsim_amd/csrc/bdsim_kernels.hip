@@ -391,7 +391,7 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
     for (int K = 0; K < nb; ++K) {
         // diagonal block: every lane loads row r (lanes 16.. duplicate 0..15)
         const int dk = dl_block(K, K);
-        double row[16];
+        double row[16], rsv[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) row[c] = Lb[dk + dl_elem(r, c)];
 #pragma unroll
@@ -406,8 +406,12 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
                 __syncthreads();
                 return false;
             }
-            const double s = sqrt(piv);
-            row[c] = r == c ? s : row[c] / s;
+            // one reciprocal square root for the pivot and the column
+            // (L_jj = piv rs, L_ij = A_ij rs: within an ulp of the square
+            // root and the division, off the dependent chain of columns)
+            const double rs = rsqrt(piv);
+            rsv[c] = rs;
+            row[c] = r == c ? piv * rs : row[c] * rs;
 #pragma unroll
             for (int q = c + 1; q < 16; ++q) row[q] -= row[c] * readlane_f64(row[c], q);
         }
@@ -423,8 +427,9 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
         // row, picked from registers.
         {
             double dinv[16], xr[16];
+            // 1 / L_jj: the pivot's reciprocal square root (unit padding rows: 1)
 #pragma unroll
-            for (int j = 0; j < 16; ++j) dinv[j] = 1.0 / readlane_f64(row[j], j);
+            for (int j = 0; j < 16; ++j) dinv[j] = 16 * K + j < n ? rsv[j] : 1.0;
 #pragma unroll
             for (int j = 15; j >= 0; --j) {
                 double sum = j == r ? 1.0 : 0.0;
