@@ -92,43 +92,61 @@ __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
 // hydrodynamics.cpp:106-136; c, dd, y are this lane's rows (in LDS). The
 // recurrences carry c[i-1], dd[i-1] and y[i+1] in registers (the same
 // operations; the rows are kept for the back substitution and the result).
+// T = float (the mixed form): the same recurrence in FP32 with one hardware
+// reciprocal per row; the vectors only seed the next warm start. The tiny
+// pivot guard becomes 1e-30.
+template <typename T>
 __device__ inline void tridiag_eigenvector(int m, lds_cptr<double> a, lds_cptr<double> b, double lambda,
-                                           lds_ptr<double> c, lds_ptr<double> dd, lds_ptr<double> y) {
+                                           lds_ptr<T> c, lds_ptr<T> dd, lds_ptr<T> y) {
+    constexpr bool F = sizeof(T) == 4;
+    const T tiny = F ? (T)1.0e-30 : (T)1.0e-300;
+    auto div = [](T x, T d) -> T {
+        if constexpr (F) return x * __builtin_amdgcn_rcpf(d);
+        else return x / d;
+    };
     const double scale = (fabs(lambda) < 1.0) ? 1.0 : fabs(lambda);
-    const double shift = lambda - 1.0e-7 * scale;
-    for (int i = 0; i < m; ++i) y[i] = 1.0 / sqrt((double)m);
+    const T shift = (T)(lambda - 1.0e-7 * scale);
+    const T y0 = (T)(1.0 / sqrt((double)m));
+    for (int i = 0; i < m; ++i) y[i] = y0;
     for (int it = 0; it < 2; ++it) {
-        double beta0 = a[0] - shift;
-        if (fabs(beta0) < 1.0e-300) beta0 = 1.0e-300;
-        double cp = (m > 1 ? b[0] / beta0 : 0.0);
-        double dp = y[0] / beta0;
+        T beta0 = (T)a[0] - shift;
+        if (fabs(beta0) < tiny) beta0 = tiny;
+        T cp = (m > 1 ? div((T)b[0], beta0) : (T)0);
+        T dp = div(y[0], beta0);
         c[0] = cp;
         dd[0] = dp;
         for (int i = 1; i < m; ++i) {
-            const double bi1 = b[i - 1];
-            double beta = (a[i] - shift) - bi1 * cp;
-            if (fabs(beta) < 1.0e-300) beta = 1.0e-300;
-            const double ci = (i < m - 1 ? b[i] / beta : 0.0);
-            const double di = (y[i] - bi1 * dp) / beta;
+            const T bi1 = (T)b[i - 1];
+            T beta = ((T)a[i] - shift) - bi1 * cp;
+            if (fabs(beta) < tiny) beta = tiny;
+            T ci, di;
+            if constexpr (F) {
+                const T r = __builtin_amdgcn_rcpf(beta);
+                ci = i < m - 1 ? (T)b[i] * r : (T)0;
+                di = (y[i] - bi1 * dp) * r;
+            } else {
+                ci = (i < m - 1 ? b[i] / beta : 0.0);
+                di = (y[i] - bi1 * dp) / beta;
+            }
             c[i] = ci;
             dd[i] = di;
             cp = ci;
             dp = di;
         }
-        double yn = dp;
+        T yn = dp;
         y[m - 1] = yn;
         for (int i = m - 2; i >= 0; --i) {
             yn = dd[i] - c[i] * yn;
             y[i] = yn;
         }
-        double nrm = 0.0;
+        T nrm = 0;
         for (int i = 0; i < m; ++i) nrm += y[i] * y[i];
         nrm = sqrt(nrm);
-        if (!(nrm > 0.0) || !isfinite(nrm)) {
-            for (int i = 0; i < m; ++i) y[i] = 1.0 / sqrt((double)m);
+        if (!(nrm > (T)0) || !isfinite(nrm)) {
+            for (int i = 0; i < m; ++i) y[i] = y0;
             return;
         }
-        for (int i = 0; i < m; ++i) y[i] /= nrm;
+        for (int i = 0; i < m; ++i) y[i] = div(y[i], nrm);
     }
 }
 
@@ -1156,15 +1174,23 @@ struct Chain {
         // Ritz vectors of the two extremes by inverse iteration (lanes 0, 1)
         if (tid < 2) {
             const lds_ptr<double> rows = tri + tid * 3 * LANCZOS_COLD;
-            tridiag_eigenvector(k, as_lds(sm.Alpha), as_lds(sm.Beta), tid == 0 ? rmin : rmax, rows,
-                                rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
+            if constexpr (MIXED) {
+                const lds_ptr<float> fr = (lds_ptr<float>)rows;
+                tridiag_eigenvector<float>(k, as_lds(sm.Alpha), as_lds(sm.Beta), tid == 0 ? rmin : rmax, fr,
+                                           fr + LANCZOS_COLD, fr + 2 * LANCZOS_COLD);
+            } else {
+                tridiag_eigenvector<double>(k, as_lds(sm.Alpha), as_lds(sm.Beta), tid == 0 ? rmin : rmax, rows,
+                                            rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
+            }
         }
         __syncthreads();
         // lift the tridiagonal eigenvectors: v = V^T y (hydrodynamics.cpp:264-273)
         double vmin[3] = {0.0, 0.0, 0.0}, vmax[3] = {0.0, 0.0, 0.0};
         if (own) {
-            const lds_cptr<double> ymn = tri + 2 * LANCZOS_COLD;
-            const lds_cptr<double> ymx = tri + 3 * LANCZOS_COLD + 2 * LANCZOS_COLD;
+            // lane 0's y row holds the smallest, lane 1's the largest
+            using Y = std::conditional_t<MIXED, float, double>;
+            const lds_cptr<Y> ymn = (lds_cptr<Y>)tri + 2 * LANCZOS_COLD;
+            const lds_cptr<Y> ymx = (lds_cptr<Y>)(tri + 3 * LANCZOS_COLD) + 2 * LANCZOS_COLD;
             for (int j = 0; j < k; ++j) {
                 const double a0 = ymn[j], a1 = ymx[j];
                 for (int d = 0; d < 3; ++d) {
