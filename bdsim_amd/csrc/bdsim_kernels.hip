@@ -363,6 +363,7 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
     const double RPI = 1.7724538509055160273;
     const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
                  C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75;
+    const double ihstar = 1.0 / hstar;
     __syncthreads();
     // padding rows n .. 16 nb - 1: zero, unit diagonal
     {
@@ -382,16 +383,19 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             for (int d = 0; d < 3; ++d) b[d] = sm.R[d][mu] - sm.R[d][nu];
             double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
             if (sq < 1.0e-12) sq = 1.0e-12;
-            const double rs = sqrt(sq);
-            const double hsbyrs = hstar / rs, rsbyhs = rs / hstar;
+            // one reciprocal square root for r, h*/r and 1/r^2 (within an ulp
+            // of the square root and divisions of hydrodynamics.cpp:30-43)
+            const double ir = rsqrt(sq), ir2 = ir * ir;
+            const double rs = sq * ir;
+            const double hsbyrs = hstar * ir, rsbyhs = rs * ihstar;
             const double hsbyrs2 = hsbyrs * hsbyrs;
             double om1, om2;
             if (rsbyhs >= TRPI) {
                 om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
-                om2 = RPI3by4 * (hsbyrs / (rs * rs)) * (1.0 - TPI * hsbyrs2);
+                om2 = RPI3by4 * (hsbyrs * ir2) * (1.0 - TPI * hsbyrs2);
             } else {
                 om1 = 1.0 - C3 * rsbyhs;
-                om2 = C7 * rsbyhs / (rs * rs);
+                om2 = C7 * rsbyhs * ir2;
             }
             // entry (3mu+p, 3nu+q); the reference's index i is the lower bead's (nu: q)
             for (int p = 0; p < 3; ++p)
