@@ -17,14 +17,15 @@
 // ranges fit.
 //
 // Stepping: trajectories are grouped by block size (64..1024 threads, from
-// N). Each group runs its own sequence of launches on its own stream with a
+// N); Cholesky chains whose factor fits in LDS (3N <= 96) form two groups of
+// their own on the dense kernel. Each group runs its own sequence of launches on its own stream with a
 // per-launch step budget; a trajectory's segment is resumed launch after
 // launch (SegState), which is bit for bit one uninterrupted integrate call, so
 // a job's result does not depend on what else is resident. poll() retires the
 // groups whose launch has finished: it collects finished trajectories,
 // completes jobs whose trajectories are all done (results copied to host),
 // frees their ranges, admits queued jobs of that group and launches again.
-// Groups of different block sizes run concurrently.
+// Groups run concurrently.
 
 #include <chrono>
 #include <deque>
@@ -86,6 +87,7 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 struct SrvJob {
     int64_t id = 0;
     int n_traj = 0, N = 0, block = 64;
+    int nb = 0;                       // > 0: Cholesky chains with the factor in LDS
     // packed per-trajectory parameters (blob/ext offsets relative to the job)
     std::vector<TrajParams> tp;
     std::vector<double> blob;
@@ -109,6 +111,7 @@ struct SrvJob {
 
 struct SrvGroup {
     int block = 64;
+    int nb = 0;                       // dense LDS kernel (k_integrate_dense<64, *, nb>)
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     bool in_flight = false;
@@ -121,7 +124,13 @@ struct SrvGroup {
     int steps = 1;
 };
 
-int group_of(int block) {
+// groups 0-4: k_integrate by block size; 5 and 6: the Cholesky chains whose
+// factor fits in LDS (3N <= 64, 3N <= 96)
+constexpr int SRV_GROUPS = 7;
+
+int group_of(int block, int nb) {
+    if (nb == 4) return 5;
+    if (nb == 6) return 6;
     switch (block) {
         case 64: return 0;
         case 128: return 1;
@@ -151,7 +160,7 @@ struct bdsim_server {
     std::deque<int64_t> queue;
     std::unordered_map<int64_t, SrvJob> jobs;
     std::deque<int64_t> completed;
-    SrvGroup grp[5];
+    SrvGroup grp[SRV_GROUPS];
     int64_t n_running = 0, n_active_traj = 0;
     int64_t bead_steps_done = 0;
 
@@ -209,7 +218,7 @@ bool srv_admit(bdsim_server* s, SrvJob& j) {
     std::vector<SegState> seg(j.n_traj, SegState{j.t0, 0, 0});
     std::vector<int32_t> st(j.n_traj, 0);
     std::vector<Counters> cz(j.n_traj, Counters{});
-    SrvGroup& gr = s->grp[group_of(j.block)];
+    SrvGroup& gr = s->grp[group_of(j.block, j.nb)];
     hipStream_t q = gr.stream;
     bool ok = hipMemcpyAsync(s->tp + slot, tps.data(), sizeof(TrajParams) * j.n_traj, hipMemcpyHostToDevice, q) == hipSuccess &&
               hipMemcpyAsync(s->R + roff, j.R0.data(), sizeof(double) * nR, hipMemcpyHostToDevice, q) == hipSuccess &&
@@ -290,7 +299,7 @@ int srv_service(bdsim_server* s, int gi) {
     // admit queued jobs of this group, in arrival order
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         SrvJob& j = s->jobs[*it];
-        if (group_of(j.block) == gi && srv_admit(s, j)) it = s->queue.erase(it);
+        if (group_of(j.block, j.nb) == gi && srv_admit(s, j)) it = s->queue.erase(it);
         else ++it;
     }
     if (gr.active.empty()) return 0;
@@ -329,9 +338,9 @@ int srv_service(bdsim_server* s, int gi) {
     long long waves = 0;
     for (const SrvGroup& o : s->grp) waves += (long long)o.active.size() * (o.block / 64);
     if (s->precision == BDSIM_PRECISION_MIXED)
-        launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, 0, waves);
+        launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves);
     else
-        launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a, 0, waves);
+        launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(gr.ev, gr.stream));
     gr.in_flight = true;
@@ -377,10 +386,12 @@ int bdsim_server_create(bdsim_server** out, int precision, int64_t hbm_bytes, in
     s->a_ext.reset(ext_cap);
     s->a_ws.reset(ws_cap / WS_ALIGN * WS_ALIGN);
     s->slot_job.assign(s->slot_cap, -1);
-    const int blocks[5] = {64, 128, 256, 512, 1024};
-    for (int i = 0; i < 5; ++i) {
+    const int blocks[SRV_GROUPS] = {64, 128, 256, 512, 1024, 64, 64};
+    const int nbs[SRV_GROUPS] = {0, 0, 0, 0, 0, 4, 6};
+    for (int i = 0; i < SRV_GROUPS; ++i) {
         SrvGroup& gr = s->grp[i];
         gr.block = blocks[i];
+        gr.nb = nbs[i];
         // about the same time per launch in every group: the pair work of a
         // chain step grows with BLOCK^2 spread over BLOCK / 64 waves
         gr.steps = std::max(1, steps_per_launch * 64 / blocks[i]);
@@ -405,6 +416,7 @@ int bdsim_server_submit(bdsim_server* s, int64_t job_id, int n_traj, int nbeads,
     j.n_traj = n_traj;
     j.N = nbeads;
     j.block = block_for(nbeads);
+    j.nb = dense_nb_for(nbeads, phys->hstar, phys->hi_method);
     TrajParams proto;
     if (int rc = pack_traj(*phys, *sim, nbeads, j.blob, j.ext, proto)) return rc;
     j.tp.assign(n_traj, proto);
@@ -430,7 +442,7 @@ int bdsim_server_poll(bdsim_server* s, int wait_us) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             before = s->completed.size();
-            for (int gi = 0; gi < 5; ++gi)
+            for (int gi = 0; gi < SRV_GROUPS; ++gi)
                 if (int rc = srv_service(s, gi)) return rc;
             if (s->completed.size() > before || wait_us <= 0) return (int)s->completed.size();
             if (s->queue.empty() && s->n_running == 0) return (int)s->completed.size();

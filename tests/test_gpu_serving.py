@@ -9,17 +9,20 @@ import pytest
 import helpers as H
 from bdsim_amd import _abi as A
 from bdsim_amd import serving as S
-from bdsim_amd.core import Rng, SimParams
+from bdsim_amd.core import DelSMethod, Rng, SimParams
 
 pytestmark = pytest.mark.gpu
 
-# (N, n_traj, steps, h*, Wi, t0): buckets of 64..512 threads, free draining,
-# a segment that does not start at 0
+# (N, n_traj, steps, h*, Wi, t0[, method]): buckets of 64..512 threads, free
+# draining, a segment that does not start at 0, and Cholesky chains on both
+# dense LDS kernels (3N <= 64, 3N <= 96) and beyond them (workspace factor)
 SPECS = [(10, 3, 57, 0.3, 1.0, 0.0), (50, 4, 120, 0.3, 10.0, 0.25), (64, 2, 33, 0.0, 0.5, 0.0),
-         (65, 2, 40, 0.25, 3.0, 0.0), (130, 2, 25, 0.2, 1.0, 1.0), (300, 1, 9, 0.3, 2.0, 0.0)]
+         (65, 2, 40, 0.25, 3.0, 0.0), (130, 2, 25, 0.2, 1.0, 1.0), (300, 1, 9, 0.3, 2.0, 0.0),
+         (12, 3, 45, 0.25, 1.0, 0.0, DelSMethod.Cholesky), (30, 2, 38, 0.25, 2.0, 0.5, DelSMethod.Cholesky),
+         (40, 1, 20, 0.2, 1.0, 0.0, DelSMethod.Cholesky)]
 
 
-def make_job(j, N, n_traj, steps, hstar, wi, t0=0.0):
+def make_job(j, N, n_traj, steps, hstar, wi, t0=0.0, method=None):
     rs = np.random.default_rng(77 + j)
     R0 = np.empty((n_traj, N, 3))
     for k in range(n_traj):
@@ -27,7 +30,10 @@ def make_job(j, N, n_traj, steps, hstar, wi, t0=0.0):
     sim = SimParams()
     sim.time_start, sim.dt = t0, 1e-3
     sim.time_end = t0 + (steps - 1) * 1e-3
-    return S.Job(j, R0, S.c5_job_physics(N, wi, hstar), sim, [Rng(100 * j + k) for k in range(n_traj)])
+    phys = S.c5_job_physics(N, wi, hstar)
+    if method is not None:
+        phys.hi_method = method
+    return S.Job(j, R0, phys, sim, [Rng(100 * j + k) for k in range(n_traj)])
 
 
 def reference(gpu, job, precision):
