@@ -2029,9 +2029,6 @@ int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double
     return 0;
 }
 
-// Register budget of the integrate kernel for 64-thread workgroups: waves per
-// SIMD the allocation is held to (BDSIM_WPE overrides; 1, 2 or 4; default 4,
-// the fastest measured at 4096 x N=50).
 // Register budget per block size set by the caller (bdsim_tune_set): waves
 // per SIMD, 0 = the library's own choice. Index: 64, 128, 256, 512.
 std::atomic<int> g_tune[4] = {{0}, {0}, {0}, {0}};
@@ -2045,25 +2042,34 @@ int tune_index(int block) {
     }
 }
 
-// More waves than 3 per SIMD of the device: the launch would not be resident
-// at once with the unconstrained register allocation.
-bool crowded(long long waves) {
+long long device_simds() {
     static const int simds = [] {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess)
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         return 4 * (cus > 0 ? cus : 256);
     }();
-    return waves > 3LL * simds;
+    return simds;
 }
 
-int wpe_choice() {
-    static int v = [] {
+// More waves than 3 per SIMD of the device: the launch would not be resident
+// at once with the unconstrained register allocation.
+bool crowded(long long waves) { return waves > 3LL * device_simds(); }
+
+// Register budget of the integrate kernel for 64-thread workgroups: waves per
+// SIMD the allocation is held to -- the largest that keeps the whole launch
+// resident: 1 (no spills) for up to one wave per SIMD of the device, 2 for up
+// to two, else 4 (the fastest measured at 4096 x N=50). BDSIM_WPE=1|2|4 in the
+// environment fixes it.
+int wpe_choice(long long waves) {
+    static const int fixed = [] {
         const char* e = std::getenv("BDSIM_WPE");
-        const int w = e ? std::atoi(e) : 4;
-        return (w == 1 || w == 2 || w == 4) ? w : 4;
+        const int w = e ? std::atoi(e) : 0;
+        return (w == 1 || w == 2 || w == 4) ? w : 0;
     }();
-    return v;
+    if (fixed) return fixed;
+    const long long simds = device_simds();
+    return waves <= simds ? 1 : waves <= 2 * simds ? 2 : 4;
 }
 
 // Cholesky chains whose factor fits the LDS of a one-wave workgroup: the
@@ -2098,7 +2104,7 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     const int tuned = ti >= 0 ? g_tune[ti].load() : 0;
     // multi-wave blocks: the 4-waves build if tuned so, or by the crowding test
     const bool four = tuned ? tuned == 4 : crowded(waves);
-    const int w = tuned ? tuned : wpe_choice();
+    const int w = tuned ? tuned : wpe_choice(waves);
     switch (block) {
         case 64:
             if (w == 4) hipLaunchKernelGGL((k_integrate<64, MIXED, 4>), grid, dim3(64), 0, stream, a);
