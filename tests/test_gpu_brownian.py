@@ -110,3 +110,41 @@ def test_cholesky_form_is_a_factor(gpu, orc):
     assert np.all(np.triu(Y, 1) == 0.0)
     assert np.all(np.diag(Y) > 0.0)
     assert np.max(np.abs(Y @ Y.T - D)) <= 1e-14 * np.max(np.abs(D)) * n
+
+
+def test_mixed_step_errors_against_the_oracle(gpu, orc):
+    """The mixed form's per-step errors on headline physics (N=50, FENE b=100,
+    h*=0.3 Chebyshev, SDK EV, shear Wi 0.1..100), each against the FP64
+    oracle: D.F at configurations reached after 100 steps (F the total
+    force), B.dW there (standard normal dW, cold bounds), and the positions
+    after those 100 steps. Bar 1e-5 each; the values are printed."""
+    N, steps = 50, 100
+    errs = {"D.F": [], "B.dW": [], "positions": []}
+    for i, wi in enumerate([0.1, 1.0, 10.0, 100.0]):
+        d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.3, ev=3, dstar=1.0, kappa=C.wi_kappa(wi))
+        p = H.Packed(d, N)
+        R0 = C.random_walk(N, 2000 + i)
+        sim = H.csim(C.sim_dict(0.0, (steps - 1) * 1e-3, 1e-3, 1e-3))
+        want, rc, _, _ = H.oracle_integrate(orc, N, R0, p.c, sim, H.rng_of(i))
+        assert rc == 0
+        out, _, status, _, _ = H.gpu_batch(gpu, [(N, R0, p.c, sim, H.rng_of(i))],
+                                           precision=A.PRECISION_MIXED)
+        assert status[0] == 0
+        errs["positions"].append(rel(out[0], want))
+        R = np.ascontiguousarray(want)
+        F = np.empty_like(R)
+        orc.bo_total_force(N, A.dptr(R), byref(p.c), A.dptr(F))
+        D = np.empty((3 * N, 3 * N))
+        orc.bo_rpy_matrix(N, A.dptr(R), 0.3, A.dptr(D))
+        DF = np.empty_like(R)
+        A.check(gpu, gpu.bdsim_rpy_apply_batch(1, N, 0.3, A.PRECISION_MIXED, A.dptr(R), A.dptr(F),
+                                               A.dptr(DF)))
+        errs["D.F"].append(rel(DF, D @ F.ravel()))
+        dW = np.ascontiguousarray(np.random.default_rng(i).normal(size=(1, N, 3)))
+        want_bw = _oracle_bx(orc, N, R, dW.ravel(), d)[0]
+        bw, st = _gpu_bx(gpu, N, np.ascontiguousarray(R[None]), dW, d, A.PRECISION_MIXED)
+        assert st[0] == A.STATUS_OK
+        errs["B.dW"].append(rel(bw, want_bw))
+    print({k: float(max(v)) for k, v in errs.items()})
+    for k, v in errs.items():
+        assert max(v) < 1e-5, (k, v)
