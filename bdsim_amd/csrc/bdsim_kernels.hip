@@ -323,12 +323,16 @@ __device__ __noinline__ D3 lower_matvec_ool(Smem<BLOCK, MIXED>& sm, gbl_ptr<doub
 // The RPY matrix, padded with unit rows to 16 nb (nb = ceil(3N/16) <= NB), is
 // kept as its lower nb(nb+1)/2 blocks of 16 x 16: block (I,J) at
 // 256 (I(I+1)/2 + J), element (r,c) of a block at 16 r + (c ^ r). The XOR puts
-// the 16 lanes of a row-wise or a column-wise access on 16 distinct bank pairs.
-template <int NB>
+// the 16 lanes of a row-wise or a column-wise access on 16 distinct banks (bank
+// pairs for doubles). The factor is FP64 in the FP64 form and FP32 in the
+// mixed form (half the LDS: more chains per CU).
+template <int NB, typename T>
 struct DenseLds {
-    double blk[NB * (NB + 1) / 2 * 256];
+    T blk[NB * (NB + 1) / 2 * 256];
     double xs[16 * NB];         // operand and result of L x
 };
+template <bool MIXED>
+using DenseT = std::conditional_t<MIXED, float, double>;
 
 __device__ __forceinline__ int dl_block(int I, int J) { return (I * (I + 1) / 2 + J) * 256; }
 __device__ __forceinline__ int dl_elem(int r, int c) { return 16 * r + (c ^ r); }
@@ -341,8 +345,26 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float readlane_any(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double readlane_any(double v, int lane) { return readlane_f64(v, lane); }
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 product on the matrix cores, acc += A B: A lane l = (row l%16,
+// k l/16), B lane l = (k l/16, column l%16). The accumulator layouts differ:
+// FP64 lane l value q = (row l/16 + 4q, column l%16), FP32 (row 4(l/16) + q,
+// column l%16); acc_row gives the row of value q.
+__device__ __forceinline__ mfma_d4 mfma16(double a, double b, mfma_d4 acc) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ mfma_f4 mfma16(float a, float b, mfma_f4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ int acc_row(int g, int q) { return sizeof(T) == 8 ? g + 4 * q : 4 * g + q; }
 
 // Dense RPY tensor (the entries of dense_factor_ool) and its lower Cholesky
 // factor, in LDS, by 16 x 16 blocks, right-looking: per block column K the
@@ -355,9 +377,10 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 // column-by-column factor up to rounding. Pivots are taken in column order,
 // so a non-positive one is reported at the same index.
 template <int BLOCK, bool MIXED, int NB>
-__device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<double> Lb, int N,
+__device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<DenseT<MIXED>> Lb, int N,
                                               double hstar) {
     static_assert(BLOCK == 64, "one wave per chain");
+    using T = DenseT<MIXED>;
     const int n = 3 * N, tid = threadIdx.x;
     const int nb = (n + 15) >> 4;
     const double RPI = 1.7724538509055160273;
@@ -370,7 +393,7 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
         const int np = 16 * nb, rows = np - n;
         for (int e = tid; e < rows * np; e += BLOCK) {
             const int i = n + e / np, j = e % np;
-            if (j <= i) Lb[dl_at(i, j)] = i == j ? 1.0 : 0.0;
+            if (j <= i) Lb[dl_at(i, j)] = i == j ? (T)1 : (T)0;
         }
     }
     // lower triangle of D: bead pairs mu > nu spread over the wave (pair
@@ -400,30 +423,30 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             // entry (3mu+p, 3nu+q); the reference's index i is the lower bead's (nu: q)
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q)
-                    Lb[dl_at(3 * mu + p, 3 * nu + q)] = om2 * b[q] * b[p] + (q == p ? om1 : 0.0);
+                    Lb[dl_at(3 * mu + p, 3 * nu + q)] = (T)(om2 * b[q] * b[p] + (q == p ? om1 : 0.0));
             nu += BLOCK;
             while (nu >= mu) { nu -= mu; ++mu; }
         }
         if (tid < N)
             for (int p = 0; p < 3; ++p)
-                for (int q = 0; q <= p; ++q) Lb[dl_at(3 * tid + p, 3 * tid + q)] = p == q ? 1.0 : 0.0;
+                for (int q = 0; q <= p; ++q) Lb[dl_at(3 * tid + p, 3 * tid + q)] = p == q ? (T)1 : (T)0;
     }
     __syncthreads();
     const int r = tid & 15, g = tid >> 4;
     for (int K = 0; K < nb; ++K) {
         // diagonal block: every lane loads row r (lanes 16.. duplicate 0..15)
         const int dk = dl_block(K, K);
-        double row[16], rsv[16];
+        T row[16], rsv[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) row[c] = Lb[dk + dl_elem(r, c)];
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
             if (16 * K + c >= n) break;      // padding: already unit rows
-            const double piv = readlane_f64(row[c], c);
+            const T piv = readlane_any(row[c], c);
             if (!(piv > 0.0)) {
                 if (tid == 0) {
                     sm.cnt.fail_index = 16 * K + c;
-                    sm.cnt.fail_value = piv;
+                    sm.cnt.fail_value = (double)piv;
                 }
                 __syncthreads();
                 return false;
@@ -431,11 +454,13 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             // one reciprocal square root for the pivot and the column
             // (L_jj = piv rs, L_ij = A_ij rs: within an ulp of the square
             // root and the division, off the dependent chain of columns)
-            const double rs = rsqrt(piv);
+            T rs;
+            if constexpr (MIXED) rs = rsqrtf(piv);
+            else rs = rsqrt(piv);
             rsv[c] = rs;
             row[c] = r == c ? piv * rs : row[c] * rs;
 #pragma unroll
-            for (int q = c + 1; q < 16; ++q) row[q] -= row[c] * readlane_f64(row[c], q);
+            for (int q = c + 1; q < 16; ++q) row[q] -= row[c] * readlane_any(row[c], q);
         }
         if (tid < 16) {
 #pragma unroll
@@ -448,30 +473,28 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
         // matrix cores; the B operand of lane group g is column 4kk+g of that
         // row, picked from registers.
         {
-            double dinv[16], xr[16];
+            T dinv[16], xr[16];
             // 1 / L_jj: the pivot's reciprocal square root (unit padding rows: 1)
 #pragma unroll
-            for (int j = 0; j < 16; ++j) dinv[j] = 16 * K + j < n ? rsv[j] : 1.0;
+            for (int j = 0; j < 16; ++j) dinv[j] = 16 * K + j < n ? rsv[j] : (T)1;
 #pragma unroll
             for (int j = 15; j >= 0; --j) {
-                double sum = j == r ? 1.0 : 0.0;
+                T sum = j == r ? (T)1 : (T)0;
 #pragma unroll
-                for (int k = j + 1; k < 16; ++k) sum -= xr[k] * readlane_f64(row[j], k);
+                for (int k = j + 1; k < 16; ++k) sum -= xr[k] * readlane_any(row[j], k);
                 xr[j] = sum * dinv[j];
             }
-            double bop[4];
+            T bop[4];
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk)
                 bop[kk] = g == 0 ? xr[4 * kk] : g == 1 ? xr[4 * kk + 1] : g == 2 ? xr[4 * kk + 2] : xr[4 * kk + 3];
             for (int I = K + 1; I < nb; ++I) {
                 const int bi = dl_block(I, K);
-                mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+                std::conditional_t<MIXED, mfma_f4, mfma_d4> acc = {0, 0, 0, 0};
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Lb[bi + dl_elem(r, 4 * kk + g)], bop[kk],
-                                                               acc, 0, 0, 0);
+                for (int kk = 0; kk < 4; ++kk) acc = mfma16(Lb[bi + dl_elem(r, 4 * kk + g)], bop[kk], acc);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) Lb[bi + dl_elem(g + 4 * q, r)] = acc[q];
+                for (int q = 0; q < 4; ++q) Lb[bi + dl_elem(acc_row<T>(g, q), r)] = acc[q];
             }
         }
         __syncthreads();
@@ -483,17 +506,17 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
             const int bIK = dl_block(I, K);
             for (int J = K + 1; J <= I; ++J) {
                 const int bIJ = dl_block(I, J), bJK = dl_block(J, K);
-                mfma_d4 acc;
+                std::conditional_t<MIXED, mfma_f4, mfma_d4> acc;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = Lb[bIJ + dl_elem(g + 4 * q, r)];
+                for (int q = 0; q < 4; ++q) acc[q] = Lb[bIJ + dl_elem(acc_row<T>(g, q), r)];
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
-                    const double a = -Lb[bIK + dl_elem(r, 4 * kk + g)];
-                    const double bv = Lb[bJK + dl_elem(r, 4 * kk + g)];
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+                    const T a = -Lb[bIK + dl_elem(r, 4 * kk + g)];
+                    const T bv = Lb[bJK + dl_elem(r, 4 * kk + g)];
+                    acc = mfma16(a, bv, acc);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) Lb[bIJ + dl_elem(g + 4 * q, r)] = acc[q];
+                for (int q = 0; q < 4; ++q) Lb[bIJ + dl_elem(acc_row<T>(g, q), r)] = acc[q];
             }
         }
         __syncthreads();
@@ -504,7 +527,7 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<do
 // y = L x with the factor in LDS (linalg.cpp:109-117: s += L[i][j] x[j] for
 // j = 0..i), one row per lane, then regrouped by bead.
 template <int BLOCK, bool MIXED, int NB>
-__device__ __noinline__ D3 lower_matvec_lds(lds_ptr<double> Lb, lds_ptr<double> xs, int N, double x0,
+__device__ __noinline__ D3 lower_matvec_lds(lds_ptr<DenseT<MIXED>> Lb, lds_ptr<double> xs, int N, double x0,
                                             double x1, double x2) {
     static_assert(BLOCK == 64, "one wave per chain");
     const int tid = threadIdx.x, n = 3 * N;
@@ -525,7 +548,7 @@ __device__ __noinline__ D3 lower_matvec_lds(lds_ptr<double> Lb, lds_ptr<double> 
             for (int J = 0; J <= I; ++J) {
                 const int base = dl_block(I, J) + 16 * rr;
                 const int cend = J == I ? rr : 15;
-                for (int c = 0; c <= cend; ++c) s += Lb[base + (c ^ rr)] * xs[16 * J + c];
+                for (int c = 0; c <= cend; ++c) s += (double)Lb[base + (c ^ rr)] * xs[16 * J + c];
             }
         }
         yv[pass] = s;
@@ -580,14 +603,15 @@ struct Chain {
     const bool hi;
     const WsLayout wl;
     gbl_ptr<double> ws;         // this trajectory's workspace
-    lds_ptr<double> dlb, dxs;   // DenseLds blocks and operand (NB > 0)
+    lds_ptr<DenseT<MIXED>> dlb; // DenseLds blocks (NB > 0)
+    lds_ptr<double> dxs;        // DenseLds operand
     int par = 0;
     int part = 0, nparts = 1;   // SPLIT: this workgroup's share of the chain
     uint32_t epoch = 0;         // SPLIT: exchanges done
     bool sync_fail = false;     // SPLIT: a barrier timed out
 
     __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
-                     double* dense_blk = nullptr, double* dense_xs = nullptr)
+                     DenseT<MIXED>* dense_blk = nullptr, double* dense_xs = nullptr)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
         ws = as_gbl(A.ws + P.ws_off);
@@ -1610,7 +1634,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)
 template <int BLOCK, bool MIXED, int NB>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1))) k_integrate_dense(KArgs a) {
     __shared__ Smem<BLOCK, MIXED> sm;
-    __shared__ DenseLds<NB> dl;
+    __shared__ DenseLds<NB, DenseT<MIXED>> dl;
     const int ti = a.list[blockIdx.x];
     const TrajParams* __restrict__ P = a.tp + ti;
     if (a.seg && a.seg[ti].done) {
