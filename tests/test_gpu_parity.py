@@ -179,6 +179,14 @@ def test_singular_configuration_fails_cleanly(gpu, golden):
     assert status[0] == A.STATUS_NOT_POSITIVE_DEFINITE and cnts[0].fail_index == 55
     assert (rngs[0].ix, rngs[0].iy, rngs[0].seed) == tuple(golden["singular_integrate/rng"])
     assert status[1] == 0 and np.all(np.isfinite(out[1]))   # the batch went on
+    # the mixed form factors this chain in FP32 (LDS kernel): the same pivot
+    # fails (FP64 value -1.28e-6, FP32 -1.18e-6)
+    out, rngs, status, cnts, _ = H.gpu_batch(gpu, [(21, C.R0_SINGULAR, p.c, sim, H.rng_of(6)),
+                                                   (21, C.random_walk(21, 3), ok.c, sim, H.rng_of(7))],
+                                             precision=A.PRECISION_MIXED)
+    assert status[0] == A.STATUS_NOT_POSITIVE_DEFINITE and cnts[0].fail_index == 55
+    assert cnts[0].fail_value < 0.0
+    assert status[1] == 0 and np.all(np.isfinite(out[1]))
 
 
 def test_uneven_batch_equals_single_runs(gpu):
