@@ -401,29 +401,35 @@ __device__ __noinline__ bool dense_factor_lds(Smem<BLOCK, MIXED>& sm, lds_ptr<De
     {
         int mu = 1, nu = tid;
         while (nu >= mu) { nu -= mu; ++mu; }
+        // the mixed form builds its FP32 factor's entries in FP32 (the
+        // separation formed in FP64, then rounded)
+        const T th = (T)hstar, tih = (T)ihstar;
         for (; mu < N;) {
-            double b[3];
-            for (int d = 0; d < 3; ++d) b[d] = sm.R[d][mu] - sm.R[d][nu];
-            double sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-            if (sq < 1.0e-12) sq = 1.0e-12;
+            T b[3];
+            for (int d = 0; d < 3; ++d) b[d] = (T)(sm.R[d][mu] - sm.R[d][nu]);
+            T sq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            if (sq < (T)1.0e-12) sq = (T)1.0e-12;
             // one reciprocal square root for r, h*/r and 1/r^2 (within an ulp
             // of the square root and divisions of hydrodynamics.cpp:30-43)
-            const double ir = rsqrt(sq), ir2 = ir * ir;
-            const double rs = sq * ir;
-            const double hsbyrs = hstar * ir, rsbyhs = rs * ihstar;
-            const double hsbyrs2 = hsbyrs * hsbyrs;
-            double om1, om2;
-            if (rsbyhs >= TRPI) {
-                om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
-                om2 = RPI3by4 * (hsbyrs * ir2) * (1.0 - TPI * hsbyrs2);
+            T ir;
+            if constexpr (MIXED) ir = rsqrtf(sq);
+            else ir = rsqrt(sq);
+            const T ir2 = ir * ir;
+            const T rs = sq * ir;
+            const T hsbyrs = th * ir, rsbyhs = rs * tih;
+            const T hsbyrs2 = hsbyrs * hsbyrs;
+            T om1, om2;
+            if (rsbyhs >= (T)TRPI) {
+                om1 = (T)RPI3by4 * hsbyrs * ((T)1 + (T)C1 * hsbyrs2);
+                om2 = (T)RPI3by4 * (hsbyrs * ir2) * ((T)1 - (T)TPI * hsbyrs2);
             } else {
-                om1 = 1.0 - C3 * rsbyhs;
-                om2 = C7 * rsbyhs * ir2;
+                om1 = (T)1 - (T)C3 * rsbyhs;
+                om2 = (T)C7 * rsbyhs * ir2;
             }
             // entry (3mu+p, 3nu+q); the reference's index i is the lower bead's (nu: q)
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q)
-                    Lb[dl_at(3 * mu + p, 3 * nu + q)] = (T)(om2 * b[q] * b[p] + (q == p ? om1 : 0.0));
+                    Lb[dl_at(3 * mu + p, 3 * nu + q)] = om2 * b[q] * b[p] + (q == p ? om1 : (T)0);
             nu += BLOCK;
             while (nu >= mu) { nu -= mu; ++mu; }
         }
