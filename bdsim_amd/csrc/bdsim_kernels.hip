@@ -826,7 +826,7 @@ struct Chain {
     // D_{nu mu} x_mu (the same block), which joins a register accumulator that
     // moves one lane up (mod N) per step, so that after step k lane j holds
     // terms for row j + k - 1. For even N the pairs at k = N/2 are met from
-    // both ends, so that step adds own-row terms only. FP32 runs of 8 steps
+    // both ends, so that step adds own-row terms only. FP32 runs of 12 steps
     // are accumulated in FP64, the moving accumulator after a permute to the
     // lane of its row. y = x + the sum (identity diagonal blocks).
     __device__ __forceinline__ void apply_pairs(const double v[3], double y[3]) {
@@ -889,11 +889,11 @@ struct Chain {
             ax += (double)fx; ay += (double)fy; az += (double)fz;
         }
         int k = K;
-        for (; k >= 8; k -= 8) {
+        for (; k >= 12; k -= 12) {
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) pair(k - u, true, fx, fy, fz);
-            flush(k - 8, fx, fy, fz);
+            for (int u = 0; u < 12; ++u) pair(k - u, true, fx, fy, fz);
+            flush(k - 12, fx, fy, fz);
         }
         if (k > 0) {
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
