@@ -767,7 +767,7 @@ struct Chain {
                 const lds_cptr<float> r0 = as_lds(sm.Rf[0]), r1 = as_lds(sm.Rf[1]), r2 = as_lds(sm.Rf[2]);
                 const lds_cptr<float> v0 = as_lds(sm.W[0]), v1 = as_lds(sm.W[1]), v2 = as_lds(sm.W[2]);
                 const float rx = r0[tid], ry = r1[tid], rz = r2[tid];
-                // FP32 pair terms summed over runs of 8 partners, then
+                // FP32 pair terms summed over runs of 16 partners, then
                 // accumulated in FP64
                 auto pair = [&](int nu, float& fx, float& fy, float& fz) {
                     const float bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
@@ -781,10 +781,10 @@ struct Chain {
                 };
                 int nu = SPLIT ? part * N / nparts : 0;
                 const int nhi = SPLIT ? (part + 1) * N / nparts : N;
-                for (; nu + 8 <= nhi; nu += 8) {
+                for (; nu + 16 <= nhi; nu += 16) {
                     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) pair(nu + k, fx, fy, fz);
+                    for (int k = 0; k < 16; ++k) pair(nu + k, fx, fy, fz);
                     ax += (double)fx;
                     ay += (double)fy;
                     az += (double)fz;
