@@ -4,6 +4,7 @@ The GPU library (bdsim_amd/csrc -> bdsim_amd/libbdsim_gpu.so) is the product.
 `load_gpu()` raises if it is missing or if no GPU is visible: nothing in this
 package falls back to a CPU implementation.
 """
+import atexit
 import ctypes
 import os
 from ctypes import POINTER, c_double, c_int, c_int32, c_int64, c_void_p
@@ -161,7 +162,29 @@ class PackedPhys:
         self.c = c
 
 
+MAX_SEGMENT_STEPS = 1.0e9
+
+
+def check_clock(t0, t1, dt):
+    """The checks the library makes before a launch (check_clock in
+    bdsim_kernels.hip): the device loop `for (t = t0; t <= t1 + dt/2; t += dt)`
+    must end, so dt is positive and finite, the ends are finite, t += dt moves
+    the clock by at least dt/2 everywhere in the segment, and the segment has at
+    most 1e9 steps. Raises ValueError otherwise."""
+    t0, t1, dt = float(t0), float(t1), float(dt)
+    if not (dt > 0.0) or not np.isfinite(dt):
+        raise ValueError(f"dt must be positive and finite, got {dt!r}")
+    if not (np.isfinite(t0) and np.isfinite(t1)):
+        raise ValueError(f"time_start and time_end must be finite, got {t0!r}, {t1!r}")
+    m = max(abs(t0), abs(t1)) + dt
+    if not ((m + dt) - m >= 0.5 * dt):
+        raise ValueError(f"dt={dt!r} is below the resolution of a clock near {m!r}")
+    if (t1 - t0) / dt > MAX_SEGMENT_STEPS:
+        raise ValueError("segment longer than 1e9 steps")
+
+
 def pack_sim(sim):
+    check_clock(sim.time_start, sim.time_end, sim.dt)
     return CSim(float(sim.time_start), float(sim.time_end), float(sim.dt),
                 float(sim.implicit_loop_tol), 1 if sim.update_center_of_mass else 0)
 
@@ -198,6 +221,7 @@ FORCE_EV = 3
 def _declare_gpu(lib):
     P = POINTER
     lib.bdsim_init.argtypes = [c_int]
+    lib.bdsim_shutdown.argtypes = []
     lib.bdsim_last_error.restype = ctypes.c_char_p
     lib.bdsim_max_beads.restype = c_int
     lib.bdsim_tune_set.argtypes = [c_int, c_int]
@@ -254,7 +278,17 @@ def load_gpu():
             raise GPUUnavailable(f"bdsim_init failed ({rc}): "
                                  f"{lib.bdsim_last_error().decode()}")
         _gpu = lib
+        atexit.register(shutdown)
     return _gpu
+
+
+def shutdown():
+    """Release the library's device state (bdsim_shutdown): registered to run
+    at interpreter exit, before the HIP runtime unloads."""
+    global _gpu
+    if _gpu is not None:
+        lib, _gpu = _gpu, None
+        lib.bdsim_shutdown()
 
 
 def check(lib, rc):

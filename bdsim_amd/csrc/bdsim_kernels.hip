@@ -1980,11 +1980,28 @@ void set_rpy(TrajParams& t, double hstar) {
     for (int j = 0; j < 6; ++j) { t.rpy_d[j] = k[j]; t.rpy_f[j] = (float)k[j]; }
 }
 
+// Steps one segment may take in one kernel run at most (the step loop runs on
+// the device, so an unbounded clock would never let the launch end).
+constexpr double MAX_SEGMENT_STEPS = 1.0e9;
+
+// A segment clock the device loop `for (t = t0; t <= t1 + dt/2; t += dt)`
+// finishes: dt positive and finite, finite ends, every t += dt within the
+// segment moving t by at least dt/2, and a bounded number of steps.
+int check_clock(double t0, double t1, double dt) {
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(BDSIM_EINVAL, "dt must be positive and finite");
+    if (!std::isfinite(t0) || !std::isfinite(t1)) return fail(BDSIM_EINVAL, "time_start and time_end must be finite");
+    const double m = std::max(std::fabs(t0), std::fabs(t1)) + dt;
+    if (!((m + dt) - m >= 0.5 * dt)) return fail(BDSIM_EINVAL, "dt is below the resolution of the segment's clock");
+    if ((t1 - t0) / dt > MAX_SEGMENT_STEPS) return fail(BDSIM_EINVAL, "segment longer than 1e9 steps");
+    return 0;
+}
+
 // Host packing of one trajectory (EV constants exactly as
 // excluded_volume.cpp:14-34, evaluated with the host libm).
 int pack_traj(const bdsim_phys& p, const bdsim_sim& s, int N, std::vector<double>& blob,
               std::vector<ExtEntry>& ext, TrajParams& t) {
     std::memset(&t, 0, sizeof t);
+    if (int rc = check_clock(s.time_start, s.time_end, s.dt)) return rc;
     if (p.spring_type < 1 || p.spring_type > 7) return fail(BDSIM_EINVAL, "bad spring type");
     if (p.ev_type < 0 || p.ev_type > 4) return fail(BDSIM_EINVAL, "bad EV type");
     if (p.hi_method < 0 || p.hi_method > 2) return fail(BDSIM_EINVAL, "bad hi_method");
@@ -2236,6 +2253,38 @@ int bdsim_init(int device) {
     return ensure_init();
 }
 
+int bdsim_shutdown(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g.ready) return 0;
+    int rc = 0;
+    if (hipSetDevice(g.device) != hipSuccess) rc = fail(BDSIM_EHIP, "hipSetDevice failed at shutdown");
+    // everything queued by this library (all its streams) ends first
+    if (hipDeviceSynchronize() != hipSuccess) rc = fail(BDSIM_EHIP, "hipDeviceSynchronize failed at shutdown");
+    DevBuf* bufs[] = {&g.tp, &g.list, &g.R, &g.ws, &g.blob, &g.ext, &g.rng, &g.status,
+                      &g.cnt, &g.log, &g.obs, &g.bar, &g.xbuf};
+    for (DevBuf* b : bufs) {
+        if (b->p) (void)hipFree(b->p);
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    if (g.jx) (void)hipFree(g.jx);
+    if (g.jy) (void)hipFree(g.jy);
+    g.jx = nullptr;
+    g.jy = nullptr;
+    for (int b = 0; b < N_BUCKETS; ++b) {
+        if (g.bstream[b]) (void)hipStreamDestroy(g.bstream[b]);
+        if (g.bevent[b]) (void)hipEventDestroy(g.bevent[b]);
+        g.bstream[b] = nullptr;
+        g.bevent[b] = nullptr;
+    }
+    if (g.uploaded) (void)hipEventDestroy(g.uploaded);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    g.uploaded = nullptr;
+    g.stream = nullptr;
+    g.ready = false;
+    return rc;
+}
+
 const char* bdsim_last_error(void) { return g_err.c_str(); }
 
 int bdsim_max_beads(void) { return MAX_BEADS; }
@@ -2468,6 +2517,7 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
 
 int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
     if (!b || !sim) return fail(BDSIM_EINVAL, "null argument");
+    if (int rc = check_clock(sim->time_start, sim->time_end, sim->dt)) return rc;
     KArgs a{};
     a.tp = b->tp;
     a.list = b->list;

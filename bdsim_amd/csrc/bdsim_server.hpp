@@ -408,7 +408,7 @@ int bdsim_server_submit(bdsim_server* s, int64_t job_id, int n_traj, int nbeads,
     if (!s || !R0 || !phys || !sim || !rng) return fail(BDSIM_EINVAL, "null argument");
     if (n_traj < 1) return fail(BDSIM_EINVAL, "n_traj must be >= 1");
     if (nbeads < 2 || nbeads > MAX_BEADS) return fail(BDSIM_EINVAL, "nbeads out of range");
-    if (!(sim->dt > 0.0)) return fail(BDSIM_EINVAL, "dt must be positive");
+    if (int rc = check_clock(sim->time_start, sim->time_end, sim->dt)) return rc;
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->jobs.count(job_id)) return fail(BDSIM_EINVAL, "job id already in use");
     SrvJob j;

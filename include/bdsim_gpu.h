@@ -168,6 +168,12 @@ typedef struct bdsim_step_log {
 
 /* Selects the device and uploads constant tables. Idempotent. */
 int bdsim_init(int device);
+/* Waits for all work the library queued, then releases its streams, events
+ * and device buffers (resident batches and servers are the caller's to
+ * destroy). A later call initialises again. No counterpart in the reference,
+ * whose state dies with the process; here device memory and HIP objects are
+ * given back before the HIP runtime and any profiler unload. */
+int bdsim_shutdown(void);
 /* Text of the last API error of the calling thread ("" if none). */
 const char* bdsim_last_error(void);
 /* Largest supported chain length (beads per trajectory). */

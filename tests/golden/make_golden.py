@@ -212,6 +212,40 @@ def main():
         names.append(case["name"])
     out["traj_names"] = np.array(names)
 
+    # 100-step trajectories of the large configurations; the oracle, bit for
+    # bit the reference here, supplies the per-step decisions
+    long_names = []
+    for case in C.long_trajectory_cases():
+        N = case["N"]
+        R0 = C.make_R0(case["R0"])
+        p = H.Packed(case["phys"], N)
+        st = H.rng_of(case["seed"])
+        R, rc = H.ref_integrate(ref, N, R0, p.c, H.csim(case["sim"]), st)
+        so = H.rng_of(case["seed"])
+        Ro, rco, cnt, lg = H.oracle_integrate(orc, N, R0, p.c, H.csim(case["sim"]), so, log_cap=200)
+        assert rc == rco == 0 and np.array_equal(R, Ro), case["name"]
+        assert (st.ix, st.iy, st.seed) == (so.ix, so.iy, so.seed)
+        key = "long/" + case["name"]
+        out[key + "/R0"] = R0
+        out[key + "/R"] = R
+        out[key + "/rng"] = np.array([st.ix, st.iy, st.seed], dtype=np.int32)
+        out[key + "/log"] = np.array([[l.lanczos_cold, l.lanczos_steps, l.cheb_L, l.cheb_attempts,
+                                       l.fallback, l.corrector_iters] for l in lg], dtype=np.int32)
+        out[key + "/t"] = np.array([l.t for l in lg])
+        long_names.append(case["name"])
+        print("long case", case["name"], "steps", cnt.steps, flush=True)
+    out["long_names"] = np.array(long_names)
+
+    # the degenerate configuration under Chebyshev with an FD tolerance no
+    # series meets: the fallback factor throws (linalg.cpp:16-24)
+    fc = {c["name"]: c for c in C.failure_cases()}["singular_cheb_fallback_not_pd"]
+    p = H.Packed(fc["phys"], 21)
+    st = H.rng_of(fc["seed"])
+    R, rc = H.ref_integrate(ref, 21, C.R0_SINGULAR, p.c, H.csim(fc["sim"]), st)
+    out["singular_cheb/rc"] = np.array(rc)
+    out["singular_cheb/message"] = np.array(ref.ref_last_error().decode())
+    out["singular_cheb/rng"] = np.array([st.ix, st.iy, st.seed], dtype=np.int32)
+
     # the singular configuration through integrate (Cholesky, one step)
     sd = C.phys_dict(spring=1, sqrtb=1000.0, hstar=C.SINGULAR_HSTAR, method=1)
     p = H.Packed(sd, 21)

@@ -40,3 +40,23 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.CCounters) == 10 * 8 + 8 + 8
     assert ctypes.sizeof(_abi.CStepLog) == 4 * 8 + 6 * 4
     assert ctypes.sizeof(_abi.CObservables) == 26 * 8
+
+
+@pytest.mark.parametrize("t0,t1,dt,ok", [
+    (0.0, 1.0, 0.01, True), (5.0, 5.0, 1e-3, True), (0.0, -1.0, 0.1, True),
+    (0.0, 1.0, 0.0, False), (0.0, 1.0, -0.01, False), (0.0, 1.0, float("nan"), False),
+    (0.0, float("inf"), 0.01, False), (float("nan"), 1.0, 0.01, False),
+    (1e12, 1e12 + 1.0, 1e-6, False),          # t += dt would not move the clock
+    (0.0, 1.0, 1e-10, False),                  # more than 1e9 steps
+])
+def test_segment_clock_checks(t0, t1, dt, ok):
+    """The host-side checks that keep the device step loop finite (the same
+    checks the library makes before any launch and returns BDSIM_EINVAL for)."""
+    class S:
+        time_start, time_end, implicit_loop_tol, update_center_of_mass = t0, t1, 1e-3, True
+    S.dt = dt
+    if ok:
+        _abi.pack_sim(S)
+    else:
+        with pytest.raises(ValueError):
+            _abi.pack_sim(S)

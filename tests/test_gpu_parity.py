@@ -249,61 +249,82 @@ def test_cholesky_factor_sizes(gpu, orc, N):
     assert status[0] == 0 and rel(out[0], want) < 1e-5
 
 
-def test_long_chains_n1000(gpu, orc):
-    """Two chains of the N=1000 workload (FENE, bending, Lennard-Jones, RPY
-    h*=0.2 Chebyshev, 16 waves per chain): 6 steps within 1e-10 of the FP64
-    oracle with the same per-step decisions (Lanczos, Chebyshev degree,
-    corrector passes), and within 1e-5 in the mixed form."""
-    N, R, phys, dt = W.long_workload(n_traj=2)
-    n = int(N[0])
-    sim = A.CSim(0.0, 5 * dt, dt, 1e-3, 1)
-    want, logs_o = [], []
-    for i in range(2):
-        Ro, rc, cnto, lo = H.oracle_integrate(orc, n, R[i], phys[i], sim, A.CRng(1, -1, i), log_cap=8)
-        assert rc == 0 and cnto.steps == 6
-        want.append(Ro)
-        logs_o.append(lo)
-    items = [(n, R[i], phys[i], sim, A.CRng(1, -1, i)) for i in range(2)]
-    out, _, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=8)
+def _long_items(golden, names):
+    cases = {c["name"]: c for c in C.long_trajectory_cases()}
+    items, keep = [], []
+    for nm in names:
+        c = cases[nm]
+        p = H.Packed(c["phys"], c["N"])
+        keep.append(p)
+        items.append((c["N"], golden[f"long/{nm}/R0"], p.c, H.csim(c["sim"]), H.rng_of(c["seed"])))
+    return items, keep
+
+
+LOG_KEYS = ("lanczos_cold", "lanczos_steps", "cheb_L", "cheb_attempts", "fallback",
+            "corrector_iters")
+
+
+@pytest.mark.parametrize("which", ["together", "n1000_split"])
+def test_long_trajectories_100_steps(gpu, golden, which):
+    """100 steps of the large configurations against the reference: two chains
+    of N=256 (four-wave workgroups, uneven-workload physics at Wi 1 and 100)
+    and two of N=1000 (sixteen waves, bending + Lennard-Jones). 'together': all
+    four in one launch, every chain on one workgroup; 'n1000_split': the two
+    N=1000 chains alone, each split over 8 workgroups. FP64 within 1e-10 with
+    the reference's per-step decisions and RNG state; mixed within 1e-5."""
+    names = [str(n) for n in golden["long_names"]]
+    if which == "n1000_split":
+        names = [n for n in names if n.startswith("c4_")]
+    items, keep = _long_items(golden, names)
+    out, rngs, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=128)
     assert np.all(status == 0)
-    for i in range(2):
-        assert rel(out[i], want[i]) <= 1e-10, (i, rel(out[i], want[i]))
-        for k in range(6):
-            a, b = logs[i * 8 + k], logs_o[i][k]
-            assert (a.lanczos_cold, a.lanczos_steps, a.cheb_L, a.cheb_attempts, a.fallback,
-                    a.corrector_iters) == (b.lanczos_cold, b.lanczos_steps, b.cheb_L,
-                                           b.cheb_attempts, b.fallback, b.corrector_iters), (i, k)
-    out, _, status, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
-    assert np.all(status == 0)
-    for i in range(2):
-        assert rel(out[i], want[i]) < 1e-5, (i, rel(out[i], want[i]))
+    worst = {}
+    for j, nm in enumerate(names):
+        want = golden[f"long/{nm}/R"]
+        r = rel(out[j], want)
+        worst[nm] = r
+        assert r <= 1e-10, (nm, r)
+        assert (rngs[j].ix, rngs[j].iy, rngs[j].seed) == tuple(golden[f"long/{nm}/rng"]), nm
+        lg_want = golden[f"long/{nm}/log"]
+        assert cnts[j].steps == len(lg_want) == 100
+        for k in range(100):
+            a = logs[j * 128 + k]
+            assert tuple(getattr(a, f) for f in LOG_KEYS) == tuple(lg_want[k]), (nm, k)
+            assert a.t == golden[f"long/{nm}/t"][k]
+    mixed, _, status_m, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    assert np.all(status_m == 0)
+    for j, nm in enumerate(names):
+        r = rel(mixed[j], golden[f"long/{nm}/R"])
+        worst[nm + "/mixed"] = r
+        assert r < 1e-5, (nm, r)
+    print("relative position error after 100 steps:", {k: f"{v:.1e}" for k, v in worst.items()})
 
 
 @pytest.mark.parametrize("n,N", [(3, 700), (40, 520)])
 def test_split_chains(gpu, orc, n, N):
     """Batches whose chains all have N > 512 and leave compute units free run
     each chain on several workgroups (8 for 3 chains, 6 for 40 on 256 CUs),
-    every D.x split by partners: 4 steps of the first chains within 1e-10 of
+    every D.x split by partners: 20 steps of the first chains within 1e-10 of
     the FP64 oracle with the same decisions, mixed within 1e-5, RNG states as
     the oracle's."""
     d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=1, zstar=0.3, dstar=0.5,
                     kappa=C.wi_kappa(0.05))
     p = H.Packed(d, N)
-    sim = H.csim(C.sim_dict(0.0, 0.003, 1e-3, 1e-3))
+    sim = H.csim(C.sim_dict(0.0, 0.019, 1e-3, 1e-3))
     R0 = [W.self_avoiding_walk(N, np.random.default_rng(50 + i)) for i in range(n)]
     items = [(N, R0[i], p.c, sim, H.rng_of(i)) for i in range(n)]
-    out, rngs, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=8)
+    out, rngs, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=24)
     assert np.all(status == 0)
     mixed, _, status_m, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
     assert np.all(status_m == 0)
     for i in range(2):
         st = H.rng_of(i)
-        want, rc, cnto, lo = H.oracle_integrate(orc, N, R0[i], p.c, sim, st, log_cap=8)
-        assert rc == 0 and cnts[i].steps == cnto.steps == 4
+        want, rc, cnto, lo = H.oracle_integrate(orc, N, R0[i], p.c, sim, st, log_cap=24)
+        assert rc == 0 and cnts[i].steps == cnto.steps == 20
         assert rel(out[i], want) <= 1e-10, (i, rel(out[i], want))
         assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st.ix, st.iy, st.seed)
-        for k in range(4):
-            a, b = logs[i * 8 + k], lo[k]
+        for k in range(20):
+            a, b = logs[i * 24 + k], lo[k]
             assert (a.lanczos_steps, a.cheb_L, a.cheb_attempts, a.corrector_iters) == \
                 (b.lanczos_steps, b.cheb_L, b.cheb_attempts, b.corrector_iters), (i, k)
         assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))

@@ -98,7 +98,7 @@ def test_submit_rejects_what_cannot_run(gpu):
             srv.submit(make_job(1, 500, 64, 10, 0.3, 1.0))
         bad = make_job(2, 10, 1, 10, 0.3, 1.0)
         bad.sim.dt = 0.0
-        with pytest.raises(RuntimeError, match="dt"):
+        with pytest.raises(ValueError, match="dt"):     # the host-side clock check
             srv.submit(bad)
         ok = make_job(3, 10, 1, 10, 0.3, 1.0)
         srv.submit(ok)

@@ -557,3 +557,101 @@ def test_free_draining_rouse_statistics(orc):
         vals.append(np.sum((R[-1] - R[0]) ** 2))
     m, se = np.mean(vals), np.std(vals, ddof=1) / np.sqrt(len(vals))
     assert abs(m - 3 * (N - 1)) < 3.5 * se
+
+
+# --------------------------------------------------------------------------
+# failure semantics (statuses instead of the reference's abort / throw)
+# --------------------------------------------------------------------------
+
+def _wlcb_force_sans_hookean(r, q0):
+    """WLC-bounded f(r) of spring.cpp:30-37, in numpy (the form the golden
+    spring forces pin)."""
+    t = (r - q0) / (1.0 - q0)
+    a = 1.0 / (1.0 - t) ** 2
+    b = 1.0 / (1.0 + t) ** 2
+    return (2.0 / (3.0 * r)) * ((a - 1.0) / 4.0 + t - q0 * (b - 1.0) / 4.0 + q0 * t)
+
+
+def test_wlcb_unbracketed_iff_bond_equation_has_no_root(orc):
+    """rtsafe's bracket test (numerics.hpp:33-36) on the transformed residual
+    says 'no root' exactly when the bond equation r (1 + dt/4 f(r)) = gamma
+    (spring.cpp:123-172) has none on the physical range: with sigma < 0 the
+    left side stays above its value near r = 0, so a small gamma has no root."""
+    dtby4, sigma = 0.25, -1.0
+    r = np.linspace(1e-6, 1.0 - 1e-6, 200001)
+    g_min = float(np.min(r * (1.0 + dtby4 * _wlcb_force_sans_hookean(r, sigma))))
+    assert g_min > 0.1
+    for gam in [0.0, 0.01, 0.5 * g_min, 0.9 * g_min]:
+        st, _, _ = _solve(orc, 7, dtby4, gam, sigma)
+        assert st == A.STATUS_RTSAFE_UNBRACKETED, gam
+    for gam in [1.1 * g_min, 2.0 * g_min, 5.0]:
+        st, rr, _ = _solve(orc, 7, dtby4, gam, sigma)
+        assert st == 0
+        res = rr * (1.0 + dtby4 * _wlcb_force_sans_hookean(rr, sigma)) - gam
+        assert abs(res) < 1e-9 * max(1.0, gam), (gam, res)
+
+
+def test_wlcb_sigma_limits(orc):
+    """sigma > 1 is the reference's explicit abort (spring.cpp:101-105); at
+    sigma = 1 the force law divides by 1 - sigma = 0, nothing is finite, and
+    rtsafe runs out of iterations (numerics.hpp:71-72)."""
+    assert _solve(orc, 7, 0.00025, 0.5, 1.2)[0] == A.STATUS_WLCB_SIGMA
+    with np.errstate(all="ignore"):
+        assert not np.any(np.isfinite(_wlcb_force_sans_hookean(np.linspace(0.01, 0.99, 50), 1.0)))
+    assert _solve(orc, 7, 0.00025, 0.5, 1.0)[0] == A.STATUS_RTSAFE_MAXIT
+
+
+@pytest.mark.parametrize("name", [c["name"] for c in C.failure_cases()])
+def test_failure_cases_status(orc, golden, name):
+    """Each failure case ends with its status on the oracle; the degenerate
+    Chebyshev case also as the reference did (golden: it threw from the
+    fallback factor at pivot 55, after the step's draws)."""
+    c = {x["name"]: x for x in C.failure_cases()}[name]
+    p = H.Packed(c["phys"], c["N"])
+    st = H.rng_of(c["seed"])
+    _, rc, cnt, _ = H.oracle_integrate(orc, c["N"], C.make_R0(c["R0"]), p.c, H.csim(c["sim"]), st)
+    assert rc == c["status"]
+    if name == "singular_cheb_fallback_not_pd":
+        assert int(golden["singular_cheb/rc"]) == 1
+        assert "pivot 55" in str(golden["singular_cheb/message"]) and cnt.fail_index == 55
+        assert (st.ix, st.iy, st.seed) == tuple(golden["singular_cheb/rng"])
+        assert cnt.fallbacks == 1
+
+
+def test_forced_fallback_and_nonconvergence_counts(orc):
+    """The golden fallback cases take the dense factor on every step after
+    growing L to MAXCHEB; the tolerance-0 case runs 100 N + 1 corrector passes
+    per step and counts each step as non-converged."""
+    cases = {c["name"]: c for c in C.trajectory_cases()}
+    for nm in ("fallback_n10_ff", "fallback_n70_gauss"):
+        c = cases[nm]
+        p = H.Packed(c["phys"], c["N"])
+        _, rc, cnt, lg = H.oracle_integrate(orc, c["N"], C.make_R0(c["R0"]), p.c, H.csim(c["sim"]),
+                                            H.rng_of(c["seed"]), log_cap=32)
+        assert rc == 0 and cnt.fallbacks == cnt.steps > 0
+        assert all(l.cheb_L == 500 and l.fallback == 1 for l in lg)
+    c = cases["nonconverged_n8_hi"]
+    p = H.Packed(c["phys"], c["N"])
+    _, rc, cnt, _ = H.oracle_integrate(orc, c["N"], C.make_R0(c["R0"]), p.c, H.csim(c["sim"]),
+                                       H.rng_of(c["seed"]))
+    assert rc == 0 and cnt.nonconverged == cnt.steps == 5
+    assert cnt.corrector_iters == 5 * (100 * c["N"] + 1)
+
+
+def test_long_n256_golden_bit_exact(orc, golden):
+    """The two N=256 100-step cases: oracle bit for bit the reference, with the
+    stored per-step decisions (the N=1000 pair was checked the same way when
+    the fixture was written)."""
+    cases = {c["name"]: c for c in C.long_trajectory_cases()}
+    for nm in [str(n) for n in golden["long_names"] if str(n).startswith("c3_")]:
+        c = cases[nm]
+        p = H.Packed(c["phys"], c["N"])
+        assert np.array_equal(golden[f"long/{nm}/R0"], C.make_R0(c["R0"]))
+        st = H.rng_of(c["seed"])
+        R, rc, cnt, lg = H.oracle_integrate(orc, c["N"], golden[f"long/{nm}/R0"], p.c,
+                                            H.csim(c["sim"]), st, log_cap=128)
+        assert rc == 0 and np.array_equal(R, golden[f"long/{nm}/R"]), nm
+        assert (st.ix, st.iy, st.seed) == tuple(golden[f"long/{nm}/rng"])
+        got = np.array([[l.lanczos_cold, l.lanczos_steps, l.cheb_L, l.cheb_attempts, l.fallback,
+                         l.corrector_iters] for l in lg])
+        assert np.array_equal(got, golden[f"long/{nm}/log"])
