@@ -228,6 +228,8 @@ def _declare_gpu(lib):
     lib.bdsim_tune_get.argtypes = [c_int]
     lib.bdsim_integrate_batch.argtypes = [c_int, _ip, _dp, P(CPhys), P(CSim), P(CRng), c_int,
                                           _ip, P(CCounters), P(CStepLog), c_int]
+    lib.bdsim_integrate_batch_static.argtypes = lib.bdsim_integrate_batch.argtypes
+    lib.bdsim_static_scratch_bytes.restype = ctypes.c_size_t
     lib.bdsim_batch_create.argtypes = [P(c_void_p), c_int, c_int, _dp, P(CPhys), _dp,
                                        P(CRng), c_int]
     lib.bdsim_batch_run.argtypes = [c_void_p, P(CSim)]

@@ -1907,6 +1907,15 @@ struct DevBuf {
 // LDS-factor Cholesky kernels
 constexpr int N_BUCKETS = 7;
 
+// The device memory of bdsim_integrate_batch_static: one static array per
+// device, 256 MiB, held from the library's load until the process ends. In
+// host code `scratch` only names the symbol; its device address on each GPU
+// comes from hipGetSymbolAddress at initialisation (g_scratch).
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)256 << 20;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+void* g_scratch[MAX_DEVICES] = {};
+
 struct Ctx {
     bool ready = false;
     int device = -1;
@@ -1954,6 +1963,11 @@ int ensure_init() {
         HIP_TRY(hipEventCreateWithFlags(&g.bevent[b], hipEventDisableTiming));
     }
     HIP_TRY(hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming));
+    if (g.device < MAX_DEVICES && !g_scratch[g.device]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch[g.device] = ptr;
+    }
     std::vector<uint32_t> jx, jy;
     build_jump_tables(jx, jy);
     HIP_TRY(hipMalloc(&g.jx, jx.size() * sizeof(uint32_t)));
@@ -2197,9 +2211,17 @@ int split_factor(int count, int total) {
 // bar: count counters; xbuf: count * split_xbuf_doubles(G) doubles.
 size_t split_xbuf_doubles(int G) { return (size_t)2 * G * 3 * 1024; }
 
+// Zeroes n words (used for the split chains' barrier counters instead of a
+// memset, so that a launch sequence holding it can be captured in a graph).
+__global__ void k_zero_u32(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 template <bool MIXED>
 int launch_split(int count, int G, hipStream_t stream, KArgs a, uint32_t* bar, double* xbuf) {
-    HIP_TRY(hipMemsetAsync(bar, 0, sizeof(uint32_t) * count, stream));
+    hipLaunchKernelGGL(k_zero_u32, dim3((count + 255) / 256), dim3(256), 0, stream, bar, count);
+    HIP_TRY(hipGetLastError());
     a.split = G;
     a.bar = bar;
     a.xbuf = xbuf;
@@ -2224,6 +2246,197 @@ int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a, int 
 
 template <typename T>
 T* as(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
+
+// Buffer sizes (bytes) of one bdsim_integrate_batch call.
+struct BatchSizes {
+    size_t tp = 0, list = 0, R = 0, ws = 0, blob = 0, ext = 0, rng = 0, status = 0, cnt = 0,
+           log = 0, bar = 0, xbuf = 0;
+};
+
+// Device buffers of one call (wherever they were carved from).
+struct BatchBufs {
+    TrajParams* tp = nullptr;
+    int32_t* list = nullptr;
+    double* R = nullptr;
+    double* ws = nullptr;
+    double* blob = nullptr;
+    ExtEntry* ext = nullptr;
+    RngState* rng = nullptr;
+    int32_t* status = nullptr;
+    Counters* cnt = nullptr;
+    StepLog* log = nullptr;
+    uint32_t* bar = nullptr;
+    double* xbuf = nullptr;
+};
+
+// Host side of one call: packed parameters, the launch order by bucket, the
+// split factor and the sizes of the device buffers.
+struct HostBatch {
+    int n_traj = 0;
+    std::vector<TrajParams> tps;
+    std::vector<double> blob;
+    std::vector<ExtEntry> ext;
+    std::vector<int32_t> order;
+    int counts[N_BUCKETS] = {};
+    int split = 1;
+    int64_t roff = 0;
+    size_t nlog = 1;
+    BatchSizes sz;
+};
+
+constexpr int BUCKET_BLOCK[N_BUCKETS] = {64, 128, 256, 512, 1024, 64, 64};
+constexpr int BUCKET_DNB[N_BUCKETS] = {0, 0, 0, 0, 0, 4, 6};
+
+size_t static_bytes_needed(const BatchSizes& z) {
+    size_t t = 0;
+    for (size_t b : {z.tp, z.list, z.R, z.ws, z.blob, z.ext, z.rng, z.status, z.cnt, z.log, z.bar, z.xbuf})
+        t += (b + 255) & ~(size_t)255;
+    return t;
+}
+
+// Checks the arguments and packs the call on the host (no device work).
+int plan_batch(int n_traj, const int32_t* nbeads, const double* R, const bdsim_phys* phys,
+               const bdsim_sim* sim, const bdsim_rng_state* rng, int precision,
+               const int32_t* status, const bdsim_step_log* log, int log_stride, HostBatch& hb) {
+    if (n_traj < 0 || !nbeads || !R || !phys || !sim || !rng || !status)
+        return fail(BDSIM_EINVAL, "null argument");
+    if (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED)
+        return fail(BDSIM_EINVAL, "bad precision");
+    hb.n_traj = n_traj;
+    if (n_traj == 0) return 0;
+    hb.tps.resize(n_traj);
+    for (int i = 0; i < n_traj; ++i) {
+        const int N = nbeads[i];
+        if (N < 2 || N > MAX_BEADS) return fail(BDSIM_EINVAL, "nbeads out of range [2, max_beads]");
+        if (int rc = pack_traj(phys[i], sim[i], N, hb.blob, hb.ext, hb.tps[i])) return rc;
+        hb.tps[i].R_off = hb.roff;
+        hb.roff += 3 * N;
+        if (log && log_stride > 0) {
+            hb.tps[i].log_off = (int64_t)i * log_stride;
+            hb.tps[i].log_cap = log_stride;
+        }
+    }
+    if (hb.blob.empty()) hb.blob.push_back(0.0);
+    if (hb.ext.empty()) hb.ext.push_back(ExtEntry{});
+    // Buckets by kernel (block size; Cholesky chains with the factor in LDS
+    // apart), each launched on its own stream so that buckets overlap on the
+    // device; within a bucket the longest chains go first (workgroups are
+    // dispatched in order, so the short ones fill in behind).
+    std::vector<int> bucket(n_traj);
+    for (int i = 0; i < n_traj; ++i) {
+        const int nb = dense_nb_for(nbeads[i], phys[i].hstar, phys[i].hi_method);
+        int b = 0;
+        while (!(BUCKET_BLOCK[b] == block_for(nbeads[i]) && BUCKET_DNB[b] == nb)) ++b;
+        bucket[i] = b;
+    }
+    for (int b = 0; b < N_BUCKETS; ++b) {
+        const size_t first = hb.order.size();
+        for (int i = 0; i < n_traj; ++i)
+            if (bucket[i] == b) { hb.order.push_back(i); ++hb.counts[b]; }
+        std::stable_sort(hb.order.begin() + first, hb.order.end(),
+                         [&](int32_t x, int32_t y) { return nbeads[x] > nbeads[y]; });
+    }
+    // Chains of N > 512 alone in the batch and too few to fill the device:
+    // each is split over G workgroups (one per CU), with G workspaces.
+    hb.split = split_factor(hb.counts[4], (int)hb.order.size());
+    int64_t wsoff = 0;
+    for (int i = 0; i < n_traj; ++i) {
+        hb.tps[i].ws_off = wsoff;
+        const int copies = (bucket[i] == 4 && hb.split > 1) ? hb.split : 1;
+        wsoff += copies * ws_layout(nbeads[i], phys[i].hstar > 0.0).total;
+    }
+    hb.nlog = (log && log_stride > 0) ? (size_t)n_traj * log_stride : 1;
+    BatchSizes& z = hb.sz;
+    z.tp = sizeof(TrajParams) * n_traj;
+    z.list = sizeof(int32_t) * n_traj;
+    z.R = sizeof(double) * hb.roff;
+    z.ws = sizeof(double) * std::max<int64_t>(wsoff, 1);
+    z.blob = sizeof(double) * hb.blob.size();
+    z.ext = sizeof(ExtEntry) * hb.ext.size();
+    z.rng = sizeof(RngState) * n_traj;
+    z.status = sizeof(int32_t) * n_traj;
+    z.cnt = sizeof(Counters) * n_traj;
+    z.log = sizeof(StepLog) * hb.nlog;
+    if (hb.split > 1) {
+        z.bar = sizeof(uint32_t) * hb.counts[4];
+        z.xbuf = sizeof(double) * hb.counts[4] * split_xbuf_doubles(hb.split);
+    }
+    return 0;
+}
+
+// Uploads, launches every bucket, copies the results back and waits.
+int run_batch(const HostBatch& hb, const BatchBufs& d, double* R, bdsim_rng_state* rng, int precision,
+              int32_t* status, bdsim_counters* counters, bdsim_step_log* log, int log_stride) {
+    static_assert(sizeof(RngState) == sizeof(bdsim_rng_state), "rng layout");
+    static_assert(sizeof(Counters) == sizeof(bdsim_counters), "counter layout");
+    static_assert(sizeof(StepLog) == sizeof(bdsim_step_log), "log layout");
+    const int n_traj = hb.n_traj;
+    HIP_TRY(hipMemcpyAsync(d.tp, hb.tps.data(), hb.sz.tp, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(d.list, hb.order.data(), hb.sz.list, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(d.R, R, hb.sz.R, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(d.blob, hb.blob.data(), hb.sz.blob, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(d.ext, hb.ext.data(), hb.sz.ext, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipMemcpyAsync(d.rng, rng, hb.sz.rng, hipMemcpyHostToDevice, g.stream));
+    if (log && log_stride > 0) {
+        // entries past a trajectory's last step read back as zeros, not as
+        // what an earlier call left in the buffer
+        const int words = (int)(hb.sz.log / sizeof(uint32_t));
+        hipLaunchKernelGGL(k_zero_u32, dim3((words + 255) / 256), dim3(256), 0, g.stream,
+                           reinterpret_cast<uint32_t*>(d.log), words);
+        HIP_TRY(hipGetLastError());
+    }
+
+    KArgs a{};
+    a.tp = d.tp;
+    a.R = d.R;
+    a.ws = d.ws;
+    a.blob = d.blob;
+    a.ext = d.ext;
+    a.rng = d.rng;
+    a.status = d.status;
+    a.cnt = d.cnt;
+    a.log = (log && log_stride > 0) ? d.log : nullptr;
+    a.jx = g.jx;
+    a.jy = g.jy;
+    a.sim_override = 0;
+    int starts[N_BUCKETS];
+    long long waves = 0;
+    for (int b = 0, st = 0; b < N_BUCKETS; ++b) {
+        starts[b] = st;
+        st += hb.counts[b];
+        waves += (long long)hb.counts[b] * (BUCKET_BLOCK[b] / 64);
+    }
+    HIP_TRY(hipEventRecord(g.uploaded, g.stream));
+    for (int b = N_BUCKETS - 1; b >= 0; --b) {    // dense and largest blocks first
+        if (hb.counts[b] == 0) continue;
+        HIP_TRY(hipStreamWaitEvent(g.bstream[b], g.uploaded, 0));
+        a.list = d.list + starts[b];
+        int rc;
+        if (b == 4 && hb.split > 1) {
+            rc = precision == BDSIM_PRECISION_MIXED
+                     ? launch_split<true>(hb.counts[b], hb.split, g.bstream[b], a, d.bar, d.xbuf)
+                     : launch_split<false>(hb.counts[b], hb.split, g.bstream[b], a, d.bar, d.xbuf);
+        } else {
+            rc = precision == BDSIM_PRECISION_MIXED
+                     ? launch_bucket<true>(BUCKET_BLOCK[b], hb.counts[b], g.bstream[b], a, BUCKET_DNB[b], waves)
+                     : launch_bucket<false>(BUCKET_BLOCK[b], hb.counts[b], g.bstream[b], a, BUCKET_DNB[b], waves);
+        }
+        if (rc) {
+            (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
+            return rc;
+        }
+        HIP_TRY(hipEventRecord(g.bevent[b], g.bstream[b]));
+        HIP_TRY(hipStreamWaitEvent(g.stream, g.bevent[b], 0));
+    }
+    HIP_TRY(hipMemcpyAsync(R, d.R, hb.sz.R, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(rng, d.rng, hb.sz.rng, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(status, d.status, hb.sz.status, hipMemcpyDeviceToHost, g.stream));
+    if (counters) HIP_TRY(hipMemcpyAsync(counters, d.cnt, hb.sz.cnt, hipMemcpyDeviceToHost, g.stream));
+    if (log && log_stride > 0) HIP_TRY(hipMemcpyAsync(log, d.log, hb.sz.log, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    (void)n_traj;
+    return 0;
+}
 
 }  // namespace
 
@@ -2310,138 +2523,89 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bd
                           int log_stride) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = ensure_init()) return rc;
-    if (n_traj < 0 || !nbeads || !R || !phys || !sim || !rng || !status)
-        return fail(BDSIM_EINVAL, "null argument");
-    if (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED)
-        return fail(BDSIM_EINVAL, "bad precision");
+    HostBatch hb;
+    if (int rc = plan_batch(n_traj, nbeads, R, phys, sim, rng, precision, status, log, log_stride, hb))
+        return rc;
     if (n_traj == 0) return 0;
-    std::vector<TrajParams> tps(n_traj);
-    std::vector<double> blob;
-    std::vector<ExtEntry> ext;
-    int64_t roff = 0;
-    for (int i = 0; i < n_traj; ++i) {
-        const int N = nbeads[i];
-        if (N < 2 || N > MAX_BEADS) return fail(BDSIM_EINVAL, "nbeads out of range [2, max_beads]");
-        if (int rc = pack_traj(phys[i], sim[i], N, blob, ext, tps[i])) return rc;
-        tps[i].R_off = roff;
-        roff += 3 * N;
-        if (log && log_stride > 0) {
-            tps[i].log_off = (int64_t)i * log_stride;
-            tps[i].log_cap = log_stride;
-        }
+    BatchBufs d;
+    if (int rc = g.tp.ensure(hb.sz.tp)) return rc;
+    if (int rc = g.list.ensure(hb.sz.list)) return rc;
+    if (int rc = g.R.ensure(hb.sz.R)) return rc;
+    if (int rc = g.ws.ensure(hb.sz.ws)) return rc;
+    if (int rc = g.blob.ensure(hb.sz.blob)) return rc;
+    if (int rc = g.ext.ensure(hb.sz.ext)) return rc;
+    if (int rc = g.rng.ensure(hb.sz.rng)) return rc;
+    if (int rc = g.status.ensure(hb.sz.status)) return rc;
+    if (int rc = g.cnt.ensure(hb.sz.cnt)) return rc;
+    if (int rc = g.log.ensure(hb.sz.log)) return rc;
+    if (hb.sz.bar) {
+        if (int rc = g.bar.ensure(hb.sz.bar)) return rc;
+        if (int rc = g.xbuf.ensure(hb.sz.xbuf)) return rc;
     }
-    if (blob.empty()) blob.push_back(0.0);
-    if (ext.empty()) ext.push_back(ExtEntry{});
-    // Buckets by kernel (block size; Cholesky chains with the factor in LDS
-    // apart), each launched on its own stream so that buckets overlap on the
-    // device; within a bucket the longest chains go first (workgroups are
-    // dispatched in order, so the short ones fill in behind).
-    std::vector<int32_t> order;
-    std::vector<int> counts(N_BUCKETS, 0);
-    const int blocks[N_BUCKETS] = {64, 128, 256, 512, 1024, 64, 64};
-    const int dnb[N_BUCKETS] = {0, 0, 0, 0, 0, 4, 6};
-    std::vector<int> bucket(n_traj);
-    for (int i = 0; i < n_traj; ++i) {
-        const int nb = dense_nb_for(nbeads[i], phys[i].hstar, phys[i].hi_method);
-        int b = 0;
-        while (!(blocks[b] == block_for(nbeads[i]) && dnb[b] == nb)) ++b;
-        bucket[i] = b;
-    }
-    for (int b = 0; b < N_BUCKETS; ++b) {
-        const size_t first = order.size();
-        for (int i = 0; i < n_traj; ++i)
-            if (bucket[i] == b) { order.push_back(i); ++counts[b]; }
-        std::stable_sort(order.begin() + first, order.end(),
-                         [&](int32_t x, int32_t y) { return nbeads[x] > nbeads[y]; });
-    }
-    // Chains of N > 512 alone in the batch and too few to fill the device:
-    // each is split over G workgroups (one per CU), with G workspaces.
-    const int split = split_factor(counts[4], (int)order.size());
-    int64_t wsoff = 0;
-    for (int i = 0; i < n_traj; ++i) {
-        tps[i].ws_off = wsoff;
-        const int copies = (bucket[i] == 4 && split > 1) ? split : 1;
-        wsoff += copies * ws_layout(nbeads[i], phys[i].hstar > 0.0).total;
-    }
-
-    if (int rc = g.tp.ensure(sizeof(TrajParams) * n_traj)) return rc;
-    if (int rc = g.list.ensure(sizeof(int32_t) * n_traj)) return rc;
-    if (int rc = g.R.ensure(sizeof(double) * roff)) return rc;
-    if (int rc = g.ws.ensure(sizeof(double) * std::max<int64_t>(wsoff, 1))) return rc;
-    if (int rc = g.blob.ensure(sizeof(double) * blob.size())) return rc;
-    if (int rc = g.ext.ensure(sizeof(ExtEntry) * ext.size())) return rc;
-    if (int rc = g.rng.ensure(sizeof(RngState) * n_traj)) return rc;
-    if (int rc = g.status.ensure(sizeof(int32_t) * n_traj)) return rc;
-    if (int rc = g.cnt.ensure(sizeof(Counters) * n_traj)) return rc;
-    const size_t nlog = (log && log_stride > 0) ? (size_t)n_traj * log_stride : 1;
-    if (int rc = g.log.ensure(sizeof(StepLog) * nlog)) return rc;
-
-    static_assert(sizeof(RngState) == sizeof(bdsim_rng_state), "rng layout");
-    static_assert(sizeof(Counters) == sizeof(bdsim_counters), "counter layout");
-    static_assert(sizeof(StepLog) == sizeof(bdsim_step_log), "log layout");
-
-    HIP_TRY(hipMemcpyAsync(g.tp.p, tps.data(), sizeof(TrajParams) * n_traj, hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(g.list.p, order.data(), sizeof(int32_t) * n_traj, hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(g.R.p, R, sizeof(double) * roff, hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(g.blob.p, blob.data(), sizeof(double) * blob.size(), hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(g.ext.p, ext.data(), sizeof(ExtEntry) * ext.size(), hipMemcpyHostToDevice, g.stream));
-    HIP_TRY(hipMemcpyAsync(g.rng.p, rng, sizeof(RngState) * n_traj, hipMemcpyHostToDevice, g.stream));
-
-    KArgs a{};
-    a.tp = as<TrajParams>(g.tp);
-    a.R = as<double>(g.R);
-    a.ws = as<double>(g.ws);
-    a.blob = as<double>(g.blob);
-    a.ext = as<ExtEntry>(g.ext);
-    a.rng = as<RngState>(g.rng);
-    a.status = as<int32_t>(g.status);
-    a.cnt = as<Counters>(g.cnt);
-    a.log = (log && log_stride > 0) ? as<StepLog>(g.log) : nullptr;
-    a.jx = g.jx;
-    a.jy = g.jy;
-    a.sim_override = 0;
-    int starts[N_BUCKETS];
-    long long waves = 0;
-    for (int b = 0, st = 0; b < N_BUCKETS; ++b) {
-        starts[b] = st;
-        st += counts[b];
-        waves += (long long)counts[b] * (blocks[b] / 64);
-    }
-    HIP_TRY(hipEventRecord(g.uploaded, g.stream));
-    for (int b = N_BUCKETS - 1; b >= 0; --b) {    // dense and largest blocks first
-        if (counts[b] == 0) continue;
-        HIP_TRY(hipStreamWaitEvent(g.bstream[b], g.uploaded, 0));
-        a.list = as<int32_t>(g.list) + starts[b];
-        int rc;
-        if (b == 4 && split > 1) {
-            if (int e = g.bar.ensure(sizeof(uint32_t) * counts[b])) return e;
-            if (int e = g.xbuf.ensure(sizeof(double) * counts[b] * split_xbuf_doubles(split))) return e;
-            uint32_t* bar = static_cast<uint32_t*>(g.bar.p);
-            double* xb = static_cast<double*>(g.xbuf.p);
-            rc = precision == BDSIM_PRECISION_MIXED ? launch_split<true>(counts[b], split, g.bstream[b], a, bar, xb)
-                                                    : launch_split<false>(counts[b], split, g.bstream[b], a, bar, xb);
-        } else {
-            rc = precision == BDSIM_PRECISION_MIXED
-                     ? launch_bucket<true>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves)
-                     : launch_bucket<false>(blocks[b], counts[b], g.bstream[b], a, dnb[b], waves);
-        }
-        if (rc) {
-            (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
-            return rc;
-        }
-        HIP_TRY(hipEventRecord(g.bevent[b], g.bstream[b]));
-        HIP_TRY(hipStreamWaitEvent(g.stream, g.bevent[b], 0));
-    }
-    HIP_TRY(hipMemcpyAsync(R, g.R.p, sizeof(double) * roff, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(rng, g.rng.p, sizeof(RngState) * n_traj, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(status, g.status.p, sizeof(int32_t) * n_traj, hipMemcpyDeviceToHost, g.stream));
-    if (counters)
-        HIP_TRY(hipMemcpyAsync(counters, g.cnt.p, sizeof(Counters) * n_traj, hipMemcpyDeviceToHost, g.stream));
-    if (log && log_stride > 0)
-        HIP_TRY(hipMemcpyAsync(log, g.log.p, sizeof(StepLog) * nlog, hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    return 0;
+    d.tp = as<TrajParams>(g.tp);
+    d.list = as<int32_t>(g.list);
+    d.R = as<double>(g.R);
+    d.ws = as<double>(g.ws);
+    d.blob = as<double>(g.blob);
+    d.ext = as<ExtEntry>(g.ext);
+    d.rng = as<RngState>(g.rng);
+    d.status = as<int32_t>(g.status);
+    d.cnt = as<Counters>(g.cnt);
+    d.log = as<StepLog>(g.log);
+    d.bar = hb.sz.bar ? as<uint32_t>(g.bar) : nullptr;
+    d.xbuf = hb.sz.bar ? as<double>(g.xbuf) : nullptr;
+    return run_batch(hb, d, R, rng, precision, status, counters, log, log_stride);
 }
+
+int bdsim_integrate_batch_static(int n_traj, const int32_t* nbeads, double* R,
+                                 const bdsim_phys* phys, const bdsim_sim* sim,
+                                 bdsim_rng_state* rng, int precision, int32_t* status,
+                                 bdsim_counters* counters, bdsim_step_log* log, int log_stride) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    // no initialisation here: it allocates (streams, jump tables)
+    if (!g.ready) return fail(BDSIM_EINVAL, "bdsim_integrate_batch_static: call bdsim_init first");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != g.device) HIP_TRY(hipSetDevice(g.device));
+    if (g.device < 0 || g.device >= MAX_DEVICES || !g_scratch[g.device])
+        return fail(BDSIM_EINVAL, "bdsim_integrate_batch_static: no scratch address for this device");
+    HostBatch hb;
+    if (int rc = plan_batch(n_traj, nbeads, R, phys, sim, rng, precision, status, log, log_stride, hb))
+        return rc;
+    if (n_traj == 0) return 0;
+    // every buffer of the call carved from the static array, 256-byte aligned
+    unsigned char* base = static_cast<unsigned char*>(g_scratch[g.device]);
+    size_t off = 0;
+    bool fits = true;
+    auto carve = [&](size_t bytes) -> void* {
+        if (bytes == 0) return nullptr;
+        off = (off + 255) & ~(size_t)255;
+        if (bytes > STATIC_SCRATCH_BYTES || off > STATIC_SCRATCH_BYTES - bytes) { fits = false; return nullptr; }
+        void* p = base + off;
+        off += bytes;
+        return p;
+    };
+    BatchBufs d;
+    d.tp = static_cast<TrajParams*>(carve(hb.sz.tp));
+    d.list = static_cast<int32_t*>(carve(hb.sz.list));
+    d.R = static_cast<double*>(carve(hb.sz.R));
+    d.ws = static_cast<double*>(carve(hb.sz.ws));
+    d.blob = static_cast<double*>(carve(hb.sz.blob));
+    d.ext = static_cast<ExtEntry*>(carve(hb.sz.ext));
+    d.rng = static_cast<RngState*>(carve(hb.sz.rng));
+    d.status = static_cast<int32_t*>(carve(hb.sz.status));
+    d.cnt = static_cast<Counters*>(carve(hb.sz.cnt));
+    d.log = static_cast<StepLog*>(carve(hb.sz.log));
+    d.bar = static_cast<uint32_t*>(carve(hb.sz.bar));
+    d.xbuf = static_cast<double*>(carve(hb.sz.xbuf));
+    if (!fits)
+        return fail(BDSIM_ENOMEM, "bdsim_integrate_batch_static: the batch needs " +
+                                      std::to_string(static_bytes_needed(hb.sz)) + " bytes of device memory, more than the " +
+                                      std::to_string(STATIC_SCRATCH_BYTES) + "-byte static scratch");
+    return run_batch(hb, d, R, rng, precision, status, counters, log, log_stride);
+}
+
+size_t bdsim_static_scratch_bytes(void) { return STATIC_SCRATCH_BYTES; }
 
 // ---- device-resident batches ---------------------------------------------
 

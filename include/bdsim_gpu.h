@@ -31,6 +31,7 @@
 #ifndef BDSIM_GPU_H
 #define BDSIM_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -210,6 +211,28 @@ int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R,
                           bdsim_rng_state* rng, int precision,
                           int32_t* status, bdsim_counters* counters,
                           bdsim_step_log* log, int log_stride);
+
+/* The same call with no device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Every
+ * device buffer the call needs -- staged positions and generator states,
+ * per-trajectory parameters, workspaces, step logs, the split chains'
+ * barriers and exchange buffers -- is carved from one static device array of
+ * bdsim_static_scratch_bytes() (256 MiB) per device, which the library holds
+ * from its load until the process ends. Arguments and results are those of
+ * bdsim_integrate_batch (the same kernels, the same bits). bdsim_init must
+ * have been called (initialisation allocates); a batch that needs more than
+ * the array returns BDSIM_ENOMEM and launches nothing. Each call overwrites
+ * the array: calls from several threads of one process are serialised by the
+ * library's lock, and any work that used the array concurrently would race
+ * on it.                                                                     */
+int bdsim_integrate_batch_static(int n_traj, const int32_t* nbeads, double* R,
+                                 const bdsim_phys* phys, const bdsim_sim* sim,
+                                 bdsim_rng_state* rng, int precision,
+                                 int32_t* status, bdsim_counters* counters,
+                                 bdsim_step_log* log, int log_stride);
+/* Size in bytes of the static array bdsim_integrate_batch_static uses. */
+size_t bdsim_static_scratch_bytes(void);
 
 /* ---- device-resident batches (for the serving loop and benchmarks) ----- */
 
