@@ -4,15 +4,17 @@ The reference runs trajectories in separate processes, each calling
 `_bdsim.integrate` for its own chain (python/bdsim/parallel.py:62-93,
 ensemble.py:106-138). On one GPU the work of thousands of chains must go out
 together, but the per-trajectory functions handed to `run_ensemble` are opaque
-Python. So `parallel_map(..., backend="gpu")` runs each item in a worker
-thread, and every `integrate` those workers make is routed here: a call parks
-its request, and when every live worker is parked (or finished) the thread that
-completes the set runs all pending requests as ONE `bdsim_integrate_batch`
-call, then wakes the others. Trajectories are independent and seeded by index,
-so the results do not depend on how calls are grouped.
+Python. So `gpu_map` runs each item in a worker thread, and every `integrate`
+those workers make is routed here: a call parks its request, and when every
+live worker is parked (or finished) -- or when the parked requests have seen no
+new arrival for `flush_after_s` -- a waiting thread runs all pending requests
+as ONE `bdsim_integrate_batch` call, then wakes the others. Trajectories are
+independent and seeded by index, so the results do not depend on how calls
+are grouped.
 """
 import ctypes
 import threading
+import time
 
 import numpy as np
 
@@ -40,12 +42,16 @@ class _Request:
 class IntegrateBatcher:
     """Groups integrate() calls of `n_workers` cooperating threads."""
 
-    def __init__(self, n_workers, precision=_abi.PRECISION_FP64, runner=None):
+    def __init__(self, n_workers, precision=_abi.PRECISION_FP64, runner=None, flush_after_s=None):
         self.cv = threading.Condition()
         self.pending = []
         self.active = int(n_workers)
         self.precision = precision
         self._runner = runner or self._run_on_gpu
+        # pending requests that saw no new arrival for this long are launched
+        # without waiting for the remaining workers (None: wait for all)
+        self.flush_after_s = flush_after_s
+        self.last_arrival = time.monotonic()
         self.batches = 0            # diagnostics: launches made
         self.batched_calls = 0
 
@@ -65,12 +71,19 @@ class IntegrateBatcher:
         req = _Request(R, phys, sim, rng)
         with self.cv:
             self.pending.append(req)
+            self.last_arrival = time.monotonic()
             batch = self._take_if_complete()
         if batch:
             self._execute(batch)
-        with self.cv:
-            while not req.done:
-                self.cv.wait()
+        while True:
+            with self.cv:
+                if req.done:
+                    break
+                batch = self._take_if_stale()
+                if batch is None:
+                    self.cv.wait(self.flush_after_s)
+                    continue
+            self._execute(batch)
         if req.nonconverged:
             from .core import _report_nonconvergence
             _report_nonconvergence(req.nonconverged, R.shape[0], sim.implicit_loop_tol)
@@ -85,6 +98,16 @@ class IntegrateBatcher:
             batch, self.pending = self.pending, []
             return batch
         return None
+
+    def _take_if_stale(self):
+        # caller holds the lock: requests that waited flush_after_s without a
+        # new arrival go out even though some workers are still busy
+        if self.flush_after_s is None or not self.pending:
+            return None
+        if time.monotonic() - self.last_arrival < self.flush_after_s:
+            return None
+        batch, self.pending = self.pending, []
+        return batch
 
     def _execute(self, batch):
         try:

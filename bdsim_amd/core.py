@@ -376,7 +376,7 @@ def integrate(R, phys, sim, rng, precision=_abi.PRECISION_FP64):
     R = _positions(R)
     from . import batcher
     b = batcher.current()
-    if b is not None:          # inside parallel_map(backend="gpu"): batched launch
+    if b is not None:          # inside gpu_map: the calls of all items go out batched
         return b.integrate(R, phys, sim, rng)
     N = R.shape[0]
     lib = _abi.load_gpu()

@@ -1,5 +1,5 @@
-"""Host-side driver logic on the CPU: the integrate batcher (with a stand-in
-runner, no GPU), parallel_map backends, parameter pickling, initial samplers."""
+"""Host-side driver logic on the CPU: the integrate batcher and gpu_map (with
+stand-in runners, no GPU), parameter pickling, initial samplers."""
 import pickle
 import threading
 
@@ -8,7 +8,6 @@ import pytest
 
 import bdsim_amd as bd
 from bdsim_amd import batcher as B
-from bdsim_amd import parallel as par
 
 
 def fake_runner(log):
@@ -73,40 +72,118 @@ def test_batcher_early_finishers_and_errors():
     assert isinstance(out[2], RuntimeError) and "pivot 2" in str(out[2])
 
 
-def _per_traj(R0, phys, sim, rng):
-    R = R0
-    for _ in range(3):
-        R = bd.integrate(R, phys, sim, rng)
-    return float(R.sum())
-
-
-def test_parallel_map_gpu_backend_routes_through_batcher(monkeypatch):
+def test_gpu_map_routes_through_batcher():
     log = []
-    monkeypatch.setattr(B.IntegrateBatcher, "_run_on_gpu", lambda self, b: fake_runner(log)(b))
     items = list(range(6))
 
     def fn(i):
-        return _per_traj(np.full((5, 3), float(i)), bd.PhysParams(), bd.SimParams(), bd.Rng(i))
+        R = np.full((5, 3), float(i))
+        rng = bd.Rng(i)
+        for _ in range(3):
+            R = bd.integrate(R, bd.PhysParams(), bd.SimParams(), rng)
+        return float(R.sum())
 
-    out = par.parallel_map(fn, items, backend="gpu")
+    out = bd.gpu_map(fn, items, runner=fake_runner(log))
     assert out == [float((i + 3) * 15) for i in items]
     assert log == [6, 6, 6]
-    with pytest.raises(ValueError):
-        par.parallel_map(fn, items, backend="cluster")
 
 
-def test_parallel_map_skip_collects_failures(monkeypatch):
-    monkeypatch.setattr(B.IntegrateBatcher, "_run_on_gpu", lambda self, b: fake_runner([])(b))
+def test_gpu_map_bounded_workers_and_errors():
+    """n_workers bounds the threads (and so the launch width); an exception
+    propagates after all workers stop, the one of the lowest item index."""
+    log = []
 
     def fn(n):
+        if n == 3:
+            raise RuntimeError(f"planted {n}")
         return bd.integrate(np.zeros((n, 3)), bd.PhysParams(), bd.SimParams(), bd.Rng(0))
 
-    out = par.parallel_map(fn, [4, 3, 5], backend="gpu", on_error="skip")
-    assert isinstance(out[1], par.Failure) and out[1].index == 1
-    ok, bad = bd.ensemble._drop_failures(out, "trajectories", max_fraction=0.5)
-    assert len(ok) == 2 and len(bad) == 1
-    with pytest.raises(RuntimeError, match="above"):
-        bd.ensemble._drop_failures(out, "trajectories")
+    out = bd.gpu_map(fn, [4, 5, 6, 7, 8], n_workers=2, runner=fake_runner(log))
+    assert [o.shape[0] for o in out] == [4, 5, 6, 7, 8] and max(log) <= 2
+    with pytest.raises(RuntimeError, match="planted 3"):
+        bd.gpu_map(fn, [4, 3, 5, 3], runner=fake_runner([]))
+    assert bd.gpu_map(fn, [], runner=fake_runner([])) == []
+
+
+def test_batcher_flushes_when_a_worker_is_slow():
+    """A request that waited flush_after_s with no new arrival goes out
+    without the worker that is still busy in Python."""
+    import time
+    log = []
+
+    def fn(i):
+        if i == 0:
+            time.sleep(0.5)                   # a slow per-trajectory function
+        return bd.integrate(np.zeros((4, 3)), bd.PhysParams(), bd.SimParams(), bd.Rng(i))
+
+    t = time.monotonic()
+    out = bd.gpu_map(fn, [0, 1, 2, 3], flush_after_s=0.02, runner=fake_runner(log))
+    assert len(out) == 4 and log[0] == 3 and sum(log) == 4
+    assert time.monotonic() - t < 5.0
+
+
+def _oracle_runner():
+    """The batcher's launch done by the CPU oracle, one request at a time (a
+    stand-in for the GPU that computes the real step)."""
+    import helpers as H
+    from bdsim_amd import _abi as A
+    from bdsim_amd.core import status_error
+    orc = H.oracle()
+
+    def run(batch):
+        for r in batch:
+            N = r.R.shape[0]
+            pk = A.PackedPhys(r.phys, N)
+            st = A.CRng(*r.rng._state())
+            R, rc, cnt, _ = H.oracle_integrate(orc, N, r.R, pk.c, A.pack_sim(r.sim), st)
+            r.rng._set_state(st.ix, st.iy, st.seed)
+            r.out = R
+            r.error = status_error(int(rc), cnt)
+    return run
+
+
+def test_gpu_map_opaque_functions_equal_serial_calls():
+    """Per-trajectory functions that make several integrate calls of different
+    segment lengths, with Python decisions between them, give through gpu_map
+    exactly what the same functions give called one after another."""
+    import helpers as H
+    from golden import cases as C
+    orc_run = _oracle_runner()
+    phys = bd.PhysParams()
+    phys.spring.type, phys.spring.sqrtb = bd.Spring.FENE, 10.0
+    phys.hstar = 0.25
+    phys.flow = bd.flows.shear(0.5)
+
+    def traj(i):
+        R = C.random_walk(10, 40 + i)
+        rng = bd.Rng(i)
+        out = []
+        t = 0.0
+        for k in range(3 + i % 3):
+            s = bd.SimParams()
+            s.time_start, s.dt = t, 1e-3
+            s.time_end = t + (5 + 3 * k + i) * 1e-3
+            R = bd.integrate(R, phys, s, rng)
+            t = s.time_end + s.dt
+            out.append(float(np.sum((R[-1] - R[0]) ** 2)))
+            if out[-1] > 1e9:               # a data-dependent branch
+                break
+        return R, rng.__getstate__(), out
+
+    items = list(range(7))
+    from bdsim_amd.batcher import IntegrateBatcher
+    serial = []
+    for i in items:                           # one at a time: a batcher of one worker
+        b = IntegrateBatcher(1, runner=orc_run)
+        b.enlist()
+        try:
+            serial.append(traj(i))
+        finally:
+            b.leave()
+    batched = bd.gpu_map(traj, items, n_workers=4, runner=orc_run)
+    for a, b in zip(serial, batched):
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+    del H
 
 
 def test_parameter_pickle_tuples_match_reference_layout():

@@ -1,17 +1,18 @@
-"""The drivers on the GPU: `backend="gpu"` batches every trajectory's integrate
-calls into shared launches and gives results identical to one-chain-at-a-time
-execution ("serial"), bit for bit; failures are dropped as Failures."""
-import os
-
+"""gpu_map on the GPU: per-trajectory Python functions whose integrate calls are
+batched into shared launches give, call by call, the FP64 oracle's positions
+(1e-10) and generator states; a function that raises propagates."""
 import numpy as np
 import pytest
 
 import bdsim_amd as bd
+import helpers as H
+from bdsim_amd import _abi as A
+from golden import cases as C
 
 pytestmark = pytest.mark.gpu
 
 
-def _phys(N=12):
+def _phys(N):
     p = bd.PhysParams()
     p.number_of_beads = N
     p.spring.type = bd.Spring.FENE
@@ -23,53 +24,65 @@ def _phys(N=12):
     return p
 
 
-def _sim(t1=0.099, dt=1e-3):
-    s = bd.SimParams()
-    s.time_start, s.time_end, s.dt = 0.0, t1, dt
-    return s
+def _segments(i):
+    """Trajectory i's segments: 2..4 calls of different lengths on one clock."""
+    out, t = [], 0.0
+    for k in range(2 + i % 3):
+        s = bd.SimParams()
+        s.time_start, s.dt = t, 1e-3
+        s.time_end = t + (4 + 5 * k + i % 4) * 1e-3
+        out.append(s)
+        t = s.time_end + s.dt
+    return out
 
 
-def test_run_ensemble_gpu_equals_serial(gpu):
-    phys, sim = _phys(), _sim()
-    a = bd.run_ensemble(phys, sim, 16, bd.final_state, seed=3, backend="serial")
-    b = bd.run_ensemble(phys, sim, 16, bd.final_state, seed=3, backend="gpu")
-    assert len(a) == len(b) == 16
+def _trajectory(i):
+    """An opaque per-trajectory function: integrate segment by segment, and
+    between calls decide in Python (stretch the next segment's start when the
+    chain is compact)."""
+    N = 12 + (i % 3) * 20
+    phys = _phys(N)
+    R = C.random_walk(N, 500 + i, 0.9)
+    rng = bd.Rng(i)
+    calls = []
+    for s in _segments(i):
+        if np.sum((R[-1] - R[0]) ** 2) < 1.0:
+            R = R * 1.01
+        R_in = R.copy()
+        st_in = rng.__getstate__()
+        R = bd.integrate(R, phys, s, rng)
+        calls.append((N, R_in, st_in, s, R.copy(), rng.__getstate__()))
+    return calls
+
+
+@pytest.mark.parametrize("n_workers", [None, 5])
+def test_gpu_map_calls_match_oracle(gpu, orc, n_workers):
+    items = list(range(16))
+    results = bd.gpu_map(_trajectory, items, n_workers=n_workers)
+    n_calls = 0
+    for calls in results:
+        for N, R_in, st_in, s, R_out, st_out in calls:
+            pk = A.PackedPhys(_phys(N), N)
+            st = A.CRng(*st_in)
+            want, rc, _, _ = H.oracle_integrate(orc, N, R_in, pk.c, A.pack_sim(s), st)
+            assert rc == 0
+            err = np.max(np.abs(R_out - want)) / np.max(np.abs(want))
+            assert err <= 1e-10, err
+            assert st_out == (st.ix, st.iy, st.seed)
+            n_calls += 1
+    assert n_calls == sum(len(_segments(i)) for i in items)
+
+
+def test_gpu_map_mixed_form_and_errors(gpu):
+    def fn(i):
+        if i == 5:
+            raise ValueError("planted")
+        R = bd.integrate(C.random_walk(20, i), _phys(20), _segments(0)[0], bd.Rng(i))
+        return R
+
+    with pytest.raises(ValueError, match="planted"):
+        bd.gpu_map(fn, range(8))
+    a = bd.gpu_map(fn, [0, 1, 2], precision=A.PRECISION_MIXED)
+    b = bd.gpu_map(fn, [0, 1, 2])
     for x, y in zip(a, b):
-        assert np.array_equal(x, y)
-    times = [0.02, 0.05, 0.1]
-    c = bd.run_ensemble(phys, sim, 6, bd.sampled_states, args=(times,), backend="serial")
-    d = bd.run_ensemble(phys, sim, 6, bd.sampled_states, args=(times,), backend="gpu",
-                        n_workers=4)
-    for x, y in zip(c, d):
-        for u, v in zip(x, y):
-            assert np.array_equal(u, v)
-
-
-def _fails_for_index(R0, phys, sim, rng):
-    if rng.__getstate__()[2] == 2:     # seed 2 fails
-        raise RuntimeError("planted failure")
-    return bd.integrate(R0, phys, sim, rng)
-
-
-def test_run_ensemble_gpu_drops_failures(gpu):
-    phys, sim = _phys(), _sim(0.019)
-    out = bd.run_ensemble(phys, sim, 12, _fails_for_index, backend="gpu", max_failed_fraction=0.5)
-    assert len(out) == 11
-
-
-def test_simulate_gpu_equals_serial(gpu, tmp_path):
-    phys, sim = _phys(10), _sim(0.05)          # n_steps = round(0.05 / 1e-3) = 50
-    out = bd.Output(directory=str(tmp_path / "ser"), write_every=10, write_forces=True)
-    r1 = bd.simulate(phys, sim, n_traj=5, seed=7, initial=bd.Initial("fene_fraenkel",
-                                                                     {"sigma": 1.0, "dQ": 2.0}),
-                     output=out, backend="serial")
-    out2 = bd.Output(directory=str(tmp_path / "gpu"), write_every=10, write_forces=True)
-    r2 = bd.simulate(phys, sim, n_traj=5, seed=7, initial=bd.Initial("fene_fraenkel",
-                                                                     {"sigma": 1.0, "dQ": 2.0}),
-                     output=out2, backend="gpu")
-    assert len(r1) == len(r2) == 5
-    for t1, t2 in zip(r1, r2):
-        assert np.array_equal(t1.step, [0, 10, 20, 30, 40, 50])
-        assert np.array_equal(t1.positions, t2.positions)
-        assert np.array_equal(t1.forces, t2.forces)
-    assert os.path.exists(os.path.join(str(tmp_path / "gpu"), "manifest.json"))
+        assert np.max(np.abs(x - y)) / np.max(np.abs(y)) < 1e-5

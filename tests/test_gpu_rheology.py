@@ -1,17 +1,23 @@
-"""Rheology on the device (python/bdsim/rheology.py): the resident path gives
-the per-trajectory path's numbers, and the paired zero-flow siblings reduce the
-variance at low Wi without bias.
+"""Shear stress of a resident ensemble (bdsim_amd.rheology): the same numbers as
+integrating each trajectory to the sample times and evaluating
+tau_xy = sum_mu (R_mu - R_c)_x F_mu,y with the oracle's total force
+(python/bdsim/rheology.py:36-51, properties.py:44-52), and paired zero-flow
+siblings that cut the variance at low Wi without bias.
 
-Tolerance: positions of the two paths are the same bit for bit (one kernel);
-the stress sums run in another order on the device, so tau_xy agrees to 1e-10
-relative of its scale.
+Tolerance: positions are within 1e-10 of the oracle over the segments; the
+stress sums run in another order on the device, so tau_xy agrees to 1e-9 of its
+scale.
 """
+from ctypes import byref
+
 import numpy as np
 import pytest
 
+import helpers as H
+from bdsim_amd import _abi as A
 from bdsim_amd import rheology
 from bdsim_amd.core import DelSMethod, EV, Flow, PhysParams, SimParams, Spring
-from bdsim_amd.ensemble import Initial
+from bdsim_amd.initial import gaussian_chain
 
 pytestmark = pytest.mark.gpu
 
@@ -32,28 +38,45 @@ def _sim(dt):
     return s
 
 
-@pytest.mark.parametrize("vr", [False, True])
-def test_resident_equals_per_trajectory(vr):
-    p = _phys(12, 0.5, hstar=0.25, spring=Spring.FENE, sqrtb=10.0, ev=EV.Gauss)
+def _oracle_stress(orc, phys, sim, N, seed, times, flow=True):
+    """tau_xy of chain `seed` at the sample times, on the oracle: one integrate
+    segment [t_k, t_{k+1}] per sample, on one generator stream."""
+    pk = A.PackedPhys(phys, N)
+    if not flow:
+        for k in range(9):
+            pk.c.flow.kappa0[k] = 0.0
+    R = gaussian_chain(N, seed=seed)
+    st = A.CRng(1, -1, seed)
+    out, t = [], sim.time_start
+    for ts in times:
+        R, rc, _, _ = H.oracle_integrate(orc, N, R, pk.c, A.CSim(t, ts, sim.dt, sim.implicit_loop_tol, 1), st)
+        assert rc == 0
+        F = np.empty_like(R)
+        orc.bo_total_force(N, A.dptr(np.ascontiguousarray(R)), byref(pk.c), A.dptr(F))
+        Rc = R - R.mean(axis=0)
+        out.append(float(np.sum(Rc[:, 0] * F[:, 1])))
+        t = ts
+    return np.array(out)
+
+
+@pytest.mark.parametrize("control", [False, True])
+def test_resident_stress_equals_oracle(gpu, orc, control):
+    N, n = 12, 6
+    p = _phys(N, 0.5, hstar=0.25, spring=Spring.FENE, sqrtb=10.0, ev=EV.Gauss)
     sim = _sim(0.01)
     times = [0.05, 0.1, 0.17]
-    a = rheology.shear_viscosity_series(p, sim, 0.5, 6, times, variance_reduction=vr, seed=3,
-                                        backend="gpu")
-    b = rheology.shear_viscosity_series(p, sim, 0.5, 6, times, variance_reduction=vr, seed=3,
-                                        backend="gpu-resident")
-    assert a.shape == b.shape == (6, 3)
-    assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a)), np.max(np.abs(a - b))
+    got = rheology.resident_stress_samples(p, sim, n, times, seed=3, control=control)
+    sheared = got[0] if control else got
+    assert sheared.shape == (n, 3)
+    for i in range(n):
+        want = _oracle_stress(orc, p, sim, N, 3 + i, times)
+        assert np.max(np.abs(sheared[i] - want)) <= 1e-9 * np.max(np.abs(want)), i
+        if control:
+            want0 = _oracle_stress(orc, p, sim, N, 3 + i, times, flow=False)
+            assert np.max(np.abs(got[1][i] - want0)) <= 1e-9 * np.max(np.abs(want0)), i
 
 
-def test_equilibrium_stress_series_resident():
-    p = _phys(8, 0.0)
-    sim = _sim(0.01)
-    a = rheology.equilibrium_stress_series(p, sim, 4, [0.1, 0.2], seed=1, backend="gpu")
-    b = rheology.equilibrium_stress_series(p, sim, 4, [0.1, 0.2], seed=1, backend="gpu-resident")
-    assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a))
-
-
-def test_variance_reduction_at_low_wi():
+def test_variance_reduction_at_low_wi(gpu):
     """Hookean N=10 at Wi = 0.03 (rate Wi sin^2(pi/2N)): the paired estimate
     is unbiased against the direct one and its scatter over trajectories is
     far smaller (the reference measured 31x in variance at Wi = 0.03)."""
@@ -62,10 +85,9 @@ def test_variance_reduction_at_low_wi():
     p = _phys(N, rate)
     sim = _sim(0.02)
     times = list(np.arange(1, 41) * 2.0)
-    direct = rheology.shear_viscosity_series(p, sim, rate, n, times, seed=11,
-                                             backend="gpu-resident").mean(axis=1)
-    paired = rheology.shear_viscosity_series(p, sim, rate, n, times, variance_reduction=True,
-                                             seed=11, backend="gpu-resident").mean(axis=1)
+    direct = rheology.resident_viscosity_samples(p, sim, rate, n, times, seed=11).mean(axis=1)
+    paired = rheology.resident_viscosity_samples(p, sim, rate, n, times, variance_reduction=True,
+                                                 seed=11).mean(axis=1)
     vd, vp = direct.var(ddof=1), paired.var(ddof=1)
     se = np.sqrt(vd / n + vp / n)
     print(f"Wi 0.03: direct {direct.mean():.2f} +/- {np.sqrt(vd / n):.2f}, paired "
