@@ -81,6 +81,7 @@ STATUS_SYNC_TIMEOUT = 7
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_MIXED_F32_FACTOR = 2
 
 
 def dptr(a):

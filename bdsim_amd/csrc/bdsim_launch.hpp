@@ -1,0 +1,28 @@
+// bdsim_launch.hpp -- launches of the integrate kernels, one definition per
+// precision form and chain-size class in its own translation unit
+// (k_integrate_{mixed,fp64}_{small,large}.hip), so that they compile in
+// parallel.
+#pragma once
+#include "bdsim_chain.hpp"
+
+namespace bdgpu {
+
+// 64-thread chains: k_integrate<64, MIXED, wpe> (wpe = waves per SIMD the
+// register allocation is held to: 1, 2 or 4), or with nb = 4 / 6 the Cholesky
+// kernel with its factor in LDS, k_integrate_dense<64, MIXED, nb, FT>, FT
+// float for the mixed form's FP32-factor variant (f32_factor), else double.
+template <bool MIXED>
+void launch_integrate_small(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
+                            const KArgs& a);
+
+// Multi-wave chains: k_integrate<block, MIXED, wpe>, block 128..1024, wpe 1 or
+// 4 (1024: 1).
+template <bool MIXED>
+void launch_integrate_large(int block, int wpe, dim3 grid, hipStream_t stream, const KArgs& a);
+
+// Chains split over a.split workgroups each: k_integrate_split<MIXED>,
+// launched cooperatively (all parts resident together).
+template <bool MIXED>
+hipError_t launch_split_kernel(dim3 grid, hipStream_t stream, KArgs& a);
+
+}  // namespace bdgpu

@@ -337,8 +337,9 @@ int srv_service(bdsim_server* s, int gi) {
     // waves of all groups' active trajectories: the groups run side by side
     long long waves = 0;
     for (const SrvGroup& o : s->grp) waves += (long long)o.active.size() * (o.block / 64);
-    if (s->precision == BDSIM_PRECISION_MIXED)
-        launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves);
+    if (is_mixed(s->precision))
+        launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves,
+                               s->precision == BDSIM_PRECISION_MIXED_F32_FACTOR);
     else
         launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves);
     HIP_TRY(hipGetLastError());
@@ -355,7 +356,7 @@ int bdsim_server_create(bdsim_server** out, int precision, int64_t hbm_bytes, in
     std::lock_guard<std::mutex> lk(g_mu);
     if (int rc = ensure_init()) return rc;
     if (!out) return fail(BDSIM_EINVAL, "null argument");
-    if (precision != BDSIM_PRECISION_FP64 && precision != BDSIM_PRECISION_MIXED)
+    if (!valid_precision(precision))
         return fail(BDSIM_EINVAL, "bad precision");
     if (hbm_bytes < ((int64_t)64 << 20)) return fail(BDSIM_EINVAL, "memory budget below 64 MiB");
     if (steps_per_launch < 1) return fail(BDSIM_EINVAL, "steps_per_launch must be >= 1");

@@ -196,9 +196,10 @@ def c5_stream(n_jobs, rate=50.0, seed=3, step_scale=1.0):
     return jobs
 
 
-def run_stream(server: Server, jobs, timeout_s=600.0):
+def run_stream(server: Server, jobs, timeout_s=600.0, progress=None):
     """Replay `jobs` in real time: each is submitted when its arrival time has
-    passed; latency is the time from arrival to its result on the host."""
+    passed; latency is the time from arrival to its result on the host.
+    `progress(done, total)` is called as jobs complete."""
     jobs = sorted(jobs, key=lambda j: j.arrival)
     by_id = {j.job_id: j for j in jobs}
     lat = {}
@@ -217,6 +218,8 @@ def run_stream(server: Server, jobs, timeout_s=600.0):
         for i in server.completed():
             results[i] = server.result(i)
             lat[i] = time.monotonic() - t0 - by_id[i].arrival
+            if progress:
+                progress(len(lat), len(jobs))
     wall = time.monotonic() - t0
     L = np.array([lat[j.job_id] for j in jobs])
     bead_steps = sum(int(c.steps) * by_id[i].N for i, r in results.items() for c in r.counters)

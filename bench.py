@@ -26,7 +26,17 @@ stream. Peak: MI355X vector FP32 157.3 TFLOP/s (mixed) / FP64 78.6 TFLOP/s.
 
 cpu_baseline: the reference core (oracle/_ref, built from the reference
 sources) -- or, where that build is absent, the FP64 C restatement -- integrating
-a bounded sample of the same workload in one process per host core.
+a bounded sample of the same workload in one process per host core (every CPU
+the process may use, at most the GPU host's share of 16), with the host's nproc
+and CPU model.
+
+mixed_errors: the mixed form against the FP64 form on the first 256 chains --
+relative errors of D.F, B.dW and positions after 100 steps, and the count of
+chain-steps whose discrete decisions (Chebyshev degree, FD attempts, corrector
+passes, fallbacks, Krylov dimension) differ.
+
+serving: the first 40 jobs of the serving stream at their full step counts,
+replayed in real time; p50 / p99 latency from arrival to result on the host.
 """
 import argparse
 import ctypes
@@ -46,7 +56,9 @@ from bdsim_amd.workloads import (N_BEADS, N_TRAJ, WI, headline_phys, kappas, she
                                  cholesky_workload, long_workload, start_configs,
                                  uneven_workload)
 
-PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}
+PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6, "mixed_f32_factor": 157.3}
+PRECISION = {"fp64": A.PRECISION_FP64, "mixed": A.PRECISION_MIXED,
+             "mixed_f32_factor": A.PRECISION_MIXED_F32_FACTOR}
 
 
 class HipEvents:
@@ -203,7 +215,9 @@ def serving_leg(n_jobs, step_scale):
     jobs = S.c5_stream(n_jobs, step_scale=step_scale)
     print(f"bench: serving {n_jobs} jobs", file=sys.stderr, flush=True)
     with S.Server(A.PRECISION_MIXED, hbm_bytes=32 << 30, steps_per_launch=256) as srv:
-        rep = S.run_stream(srv, jobs)
+        rep = S.run_stream(srv, jobs, timeout_s=900.0,
+                           progress=lambda k, n: print(f"bench: serving {k}/{n} jobs done",
+                                                       file=sys.stderr, flush=True))
     bad = sum(int(np.sum(r.status != 0)) for r in rep["results"].values())
     return {"p50_s": rep["p50_s"], "p99_s": rep["p99_s"], "makespan_s": rep["makespan_s"],
             "bead_steps": rep["bead_steps"], "bead_steps_per_s": rep["bead_steps_per_s"],
@@ -211,8 +225,10 @@ def serving_leg(n_jobs, step_scale):
             "sample": (f"first {n_jobs} jobs of the serving stream (Poisson 50 jobs/s, seed 3; "
                        f"N log-uniform [10,500], 1..64 trajectories, Wi log-uniform [0.1,100], "
                        f"h* uniform [0,0.3], FENE b=100, Gauss EV, dt=1e-3) with n_steps "
-                       f"log-uniform [1e3,1e5] scaled by {step_scale:g}; mixed form, "
-                       f"arrivals replayed in real time, latency from arrival to host result")}
+                       f"log-uniform [1e3,1e5]"
+                       + ("" if step_scale == 1.0 else f" scaled by {step_scale:g}")
+                       + "; mixed form, arrivals replayed in real time, latency from arrival to "
+                         "host result")}
 
 
 # ---- CPU baseline ----------------------------------------------------------
@@ -239,12 +255,34 @@ def _cpu_worker(args):
     return time.perf_counter() - t0
 
 
+def host_cpu():
+    """(nproc, CPUs this process may run on, model name) of the host."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return os.cpu_count() or 1, usable, model
+
+
+# Worker processes the GPU host lets one job run beside its GPU (its CPU share).
+HOST_CPU_SHARE = 16
+
+
 def cpu_baseline(n_traj_sample=768, steps=400):
     from concurrent.futures import ProcessPoolExecutor
     kind = "reference" if os.path.exists(os.path.join(REPO, "oracle", "_ref", "libbdref.so")) else "port"
     if kind == "port" and not os.path.exists(os.path.join(REPO, "oracle", "_build", "liboracle.so")):
         return None
-    cores = min(os.cpu_count() or 1, 16, n_traj_sample)
+    nproc, usable, model = host_cpu()
+    cores = min(usable, HOST_CPU_SHARE, n_traj_sample)
     chunks = [list(range(k, n_traj_sample, cores)) for k in range(cores)]
     t0 = time.perf_counter()
     with ProcessPoolExecutor(max_workers=cores) as ex:
@@ -252,6 +290,9 @@ def cpu_baseline(n_traj_sample=768, steps=400):
     wall = time.perf_counter() - t0
     bead_steps = n_traj_sample * N_BEADS * steps
     return {"value": bead_steps / wall, "unit": "bead-steps/s", "cores": cores, "kind": kind,
+            "host": {"nproc": nproc, "usable_cpus": usable, "model": model,
+                     "note": (f"one process per core on {cores} cores: every CPU this process "
+                              f"may use, up to the GPU host's share of {HOST_CPU_SHARE} per GPU")},
             "sample": (f"{n_traj_sample} of the 4096 chains (same physics, seeds, starts), "
                        f"{steps} steps each in one integrate call, one process per core; "
                        f"{sum(busy):.1f} core-seconds of CPU work, wall {wall:.2f} s"
@@ -371,7 +412,43 @@ def mixed_errors(lib, n_traj=256, steps=100):
             "against": "fp64 form",
             "D.F": summary(out["mixed"][0], out["fp64"][0]),
             "B.dW": summary(out["mixed"][1], out["fp64"][1]),
-            "positions": summary(Rm, Rf)}
+            "positions": summary(Rm, Rf),
+            "decision_flips": decision_flips(lib, R0, K, steps)}
+
+
+def decision_flips(lib, R0, K, steps):
+    """Per chain-step discrete decisions of the two forms on the same chains
+    (same starts, seeds and steps): Chebyshev degree L, FD attempts, corrector
+    passes, Cholesky fallbacks and Krylov dimension -- counted where the mixed
+    form's differs from the FP64 form's -- and the largest relative difference
+    of the Chebyshev interval ends lmin, lmax (the Ritz bounds they come from
+    are multisected in FP32 in the mixed form)."""
+    n_traj = R0.shape[0]
+    logs = {}
+    for form, prec in (("fp64", A.PRECISION_FP64), ("mixed", A.PRECISION_MIXED)):
+        ph = []
+        for i in range(n_traj):
+            c = headline_phys()
+            c.flow.kappa0[1] = K[i, 1]
+            ph.append(c)
+        nb = np.full(n_traj, N_BEADS, dtype=np.int32)
+        R = np.array(R0, dtype=np.float64).reshape(-1)       # a copy: the call writes R back
+        sims = (A.CSim * n_traj)(*[A.CSim(0.0, (steps - 1) * 1e-3, 1e-3, 1e-3, 1)] * n_traj)
+        rng = (A.CRng * n_traj)(*[A.CRng(1, -1, i) for i in range(n_traj)])
+        st = np.zeros(n_traj, dtype=np.int32)
+        lg = (A.CStepLog * (n_traj * steps))()
+        A.check(lib, lib.bdsim_integrate_batch(n_traj, A.iptr(nb), A.dptr(R), (A.CPhys * n_traj)(*ph),
+                                               sims, rng, prec, A.iptr(st), None, lg, steps))
+        assert np.all(st == 0)
+        logs[form] = np.array([(l.cheb_L, l.cheb_attempts, l.corrector_iters, l.fallback,
+                                l.lanczos_steps, l.lmin, l.lmax) for l in lg]).reshape(n_traj, steps, 7)
+    a, b = logs["mixed"], logs["fp64"]
+    out = {"chain_steps": int(n_traj * steps)}
+    for k, name in enumerate(("cheb_L", "fd_attempts", "corrector_passes", "fallbacks", "lanczos_dim")):
+        out[name] = int(np.sum(a[:, :, k] != b[:, :, k]))
+    out["chains_with_any_flip"] = int(np.sum(np.any(a[:, :, :5] != b[:, :, :5], axis=(1, 2))))
+    out["bounds_rel_diff_max"] = float(np.max(np.abs(a[:, :, 5:] - b[:, :, 5:]) / np.abs(b[:, :, 5:])))
+    return out
 
 
 def main():
@@ -384,7 +461,7 @@ def main():
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=40)
-    ap.add_argument("--serving-step-scale", type=float, default=0.02)
+    ap.add_argument("--serving-step-scale", type=float, default=1.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--workload", choices=["headline", "uneven", "long", "cholesky", "c1"],
@@ -404,7 +481,7 @@ def main():
         lib = A.load_gpu()
         out = {"workload": args.workload, "steps": args.steps, "warmup": args.warmup}
         for form in args.forms.split(","):
-            prec = A.PRECISION_MIXED if form == "mixed" else A.PRECISION_FP64
+            prec = PRECISION[form]
             print(f"bench: {args.workload} {form}", file=sys.stderr, flush=True)
             if args.workload == "c1":
                 out[form] = run_c1(prec)
@@ -428,7 +505,7 @@ def main():
     K = kappas(args.traj)
     res = {}
     for form in args.forms.split(","):
-        prec = A.PRECISION_MIXED if form == "mixed" else A.PRECISION_FP64
+        prec = PRECISION[form]
         print(f"bench: {form} form, {args.traj} chains, warmup {args.warmup} + {args.steps} steps",
               file=sys.stderr, flush=True)
         res[form] = run_gpu(lib, prec, args.steps, args.warmup, R0, K, ev)

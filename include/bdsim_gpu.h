@@ -49,9 +49,15 @@ enum { BDSIM_HI_CHEBYSHEV = 0, BDSIM_HI_CHOLESKY = 1, BDSIM_HI_EXACT_SQRT = 2 };
 
 /* Precision forms of the device step. */
 enum { BDSIM_PRECISION_FP64 = 0,   /* plain FP64: parity form                */
-       BDSIM_PRECISION_MIXED = 1 };/* FP32 RPY/Chebyshev/EV pair sums with   */
+       BDSIM_PRECISION_MIXED = 1,  /* FP32 RPY/Chebyshev/EV pair sums with   */
                                    /* FP64 accumulation; FP64 Lanczos bounds, */
                                    /* Cholesky, connector solve, update      */
+       BDSIM_PRECISION_MIXED_F32_FACTOR = 2 };
+                                   /* the mixed form with the Cholesky factor */
+                                   /* of chains with 3N <= 96 held in FP32    */
+                                   /* (half the LDS: 5 chains per CU instead  */
+                                   /* of 3); a pivot that fails in FP32 is    */
+                                   /* factored again in FP64                  */
 
 /* Per-trajectory status codes. */
 enum {

@@ -179,14 +179,18 @@ def test_singular_configuration_fails_cleanly(gpu, golden):
     assert status[0] == A.STATUS_NOT_POSITIVE_DEFINITE and cnts[0].fail_index == 55
     assert (rngs[0].ix, rngs[0].iy, rngs[0].seed) == tuple(golden["singular_integrate/rng"])
     assert status[1] == 0 and np.all(np.isfinite(out[1]))   # the batch went on
-    # the mixed form factors this chain in FP32 (LDS kernel): the same pivot
-    # fails (FP64 value -1.28e-6, FP32 -1.18e-6)
-    out, rngs, status, cnts, _ = H.gpu_batch(gpu, [(21, C.R0_SINGULAR, p.c, sim, H.rng_of(6)),
-                                                   (21, C.random_walk(21, 3), ok.c, sim, H.rng_of(7))],
-                                             precision=A.PRECISION_MIXED)
-    assert status[0] == A.STATUS_NOT_POSITIVE_DEFINITE and cnts[0].fail_index == 55
-    assert cnts[0].fail_value < 0.0
-    assert status[1] == 0 and np.all(np.isfinite(out[1]))
+    fp64_pivot = cnts[0].fail_value
+    # the mixed form factors in FP64 (the same pivot value); its FP32-factor
+    # variant fails the FP32 pivot (-1.18e-6), factors again in FP64 (column by
+    # column, in the workspace) and reports that pivot: the same to rounding
+    # of the unit-sized entries it cancels from
+    for prec in (A.PRECISION_MIXED, A.PRECISION_MIXED_F32_FACTOR):
+        out, rngs, status, cnts, _ = H.gpu_batch(gpu, [(21, C.R0_SINGULAR, p.c, sim, H.rng_of(6)),
+                                                       (21, C.random_walk(21, 3), ok.c, sim, H.rng_of(7))],
+                                                 precision=prec)
+        assert status[0] == A.STATUS_NOT_POSITIVE_DEFINITE and cnts[0].fail_index == 55
+        assert abs(cnts[0].fail_value - fp64_pivot) <= 1e-13, (prec, cnts[0].fail_value)
+        assert status[1] == 0 and np.all(np.isfinite(out[1]))
 
 
 def test_uneven_batch_equals_single_runs(gpu):
@@ -231,7 +235,8 @@ def test_cholesky_factor_sizes(gpu, orc, N):
     """The Cholesky method across the sizes whose factor is held in LDS (3N up
     to 64 and up to 96: padding rows, whole padding blocks, exact fit) and just
     beyond (N=33, factor in the workspace): 30 steps within 1e-10 of the FP64
-    oracle with the same decisions; the mixed form within 1e-5."""
+    oracle with the same decisions; the mixed form (FP64 factor) and its
+    FP32-factor variant within 1e-5."""
     d = C.phys_dict(spring=2, sqrtb=np.sqrt(50.0), hstar=0.25, method=1, ev=1, zstar=1.0,
                     dstar=0.5, kappa=[0.5, 0, 0, 0, -0.5, 0, 0, 0, 0])
     p = H.Packed(d, N)
@@ -244,9 +249,10 @@ def test_cholesky_factor_sizes(gpu, orc, N):
     assert rel(out[0], want) <= 1e-10, rel(out[0], want)
     for k in range(30):
         assert logs[k].corrector_iters == lo[k].corrector_iters, k
-    out, _, status, _, _ = H.gpu_batch(gpu, [(N, R0, p.c, sim, H.rng_of(N))],
-                                       precision=A.PRECISION_MIXED)
-    assert status[0] == 0 and rel(out[0], want) < 1e-5
+    for prec in (A.PRECISION_MIXED, A.PRECISION_MIXED_F32_FACTOR):
+        out, _, status, cnts, logs = H.gpu_batch(gpu, [(N, R0, p.c, sim, H.rng_of(N))],
+                                                 precision=prec, log_stride=32)
+        assert status[0] == 0 and rel(out[0], want) < 1e-5, (prec, rel(out[0], want))
 
 
 def _long_items(golden, names):
