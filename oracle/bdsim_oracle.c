@@ -5,7 +5,8 @@
  * reference expression by expression -- same association, same sequential sums,
  * same libm calls -- so that, built with gcc at -O2 for baseline x86-64 (no FMA
  * contraction available), it reproduces the reference C++ core bit for bit.
- * tests/test_oracle_vs_ref.py checks exactly that against oracle/_ref.
+ * tests/test_oracle.py checks that against the golden arrays written from
+ * oracle/_ref (test_trajectories_golden_bit_exact, test_forces_golden_bit_exact).
  *
  * Citations are to IsaacPincus/bdsim.
  */

@@ -117,3 +117,22 @@ def test_stream_replay_reports_latency(gpu):
     want = sum(j.bead_steps() for j in jobs)
     assert rep["bead_steps"] == want
     assert all(np.all(r.status == 0) for r in rep["results"].values())
+
+
+def test_served_fp64_jobs_match_oracle(gpu, orc):
+    """Jobs served together in the resident batch (FP64 form), cut into short
+    launches, against the CPU oracle integrating each trajectory's segment:
+    positions within 1e-10 relative, the same generator states."""
+    jobs = [make_job(j, *s) for j, s in enumerate(SPECS) if s[0] <= 65]
+    with S.Server(A.PRECISION_FP64, hbm_bytes=1 << 30, steps_per_launch=16) as srv:
+        got = S.serve(srv, jobs)
+    for job in jobs:
+        res = got[job.job_id]
+        pk = A.PackedPhys(job.phys, job.N)
+        for k in range(job.n_traj):
+            st = A.CRng(*job.rngs[k]._state())
+            want, rc, _, _ = H.oracle_integrate(orc, job.N, job.R0[k], pk.c, A.pack_sim(job.sim), st)
+            assert rc == 0 and res.status[k] == 0
+            err = np.max(np.abs(res.R[k] - want)) / np.max(np.abs(want))
+            assert err <= 1e-10, (job.job_id, k, err)
+            assert res.rng_states[k] == (st.ix, st.iy, st.seed)
