@@ -55,21 +55,31 @@ struct KArgs {
     double* xbuf;
 };
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 template <int BLOCK, bool MIXED>
 struct Smem {
     static constexpr int NW = BLOCK / 64;
     // single-wave chains walk bead pairs (mu, mu + k) on doubled FP32 copies
     static constexpr bool PAIRS = MIXED && BLOCK == 64;
-    static constexpr int FB = MIXED ? (PAIRS ? 2 * BLOCK : BLOCK) : 1;
+    static constexpr int FB = (MIXED && !PAIRS) ? BLOCK : 1;
+    static constexpr int PB = PAIRS ? 2 * BLOCK : 1;
     static constexpr int DB = MIXED ? 1 : BLOCK;   // FP64 operand (FP64 form)
     double R[3][BLOCK];         // configuration the mobility is evaluated at
     double P[3][BLOCK];         // force configuration; corrector positions
     double X[3][DB];            // mat-vec operand
-    float Rf[3][FB];            // mixed form: positions relative to bead 0
+    float Rf[3][FB];            // mixed form, several waves: positions relative to bead 0
+    // mixed form, one wave (the pair walk): bead mu's FP32 position relative
+    // to bead 0 and the operand's x in Pa[mu], the operand's y, z in Pb[mu],
+    // both doubled (entry mu + N repeats mu), so that one 16-byte and one
+    // 8-byte read at an immediate offset fetch a partner
+    f32x4 Pa[PB];
     double B[4][BLOCK];         // bond-level arrays (unit bonds + lengths, bonds)
     union {
         double V[3][BLOCK];     // neighbour exchange; triangular-product operand
         float W[3][2 * BLOCK];  // mixed form: mat-vec operand, EV positions
+        f32x2 Pb[PB];           // the pair walk's operand y, z
     };
     double Red[2 * NW];
     double Alpha[LANCZOS_COLD];
@@ -646,13 +656,18 @@ struct Chain {
         __syncthreads();
         for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
         __syncthreads();
-        if constexpr (MIXED) {
+        if constexpr (S::PAIRS) {
+            if (own) {
+                const float rx = (float)(sm.R[0][tid] - sm.R[0][0]);
+                const float ry = (float)(sm.R[1][tid] - sm.R[1][0]);
+                const float rz = (float)(sm.R[2][tid] - sm.R[2][0]);
+                sm.Pa[tid].x = rx; sm.Pa[tid].y = ry; sm.Pa[tid].z = rz;
+                sm.Pa[tid + N].x = rx; sm.Pa[tid + N].y = ry; sm.Pa[tid + N].z = rz;
+            }
+            __syncthreads();
+        } else if constexpr (MIXED) {
             if (own)
-                for (int d = 0; d < 3; ++d) {
-                    const float r = (float)(sm.R[d][tid] - sm.R[d][0]);
-                    sm.Rf[d][tid] = r;
-                    if constexpr (S::PAIRS) sm.Rf[d][tid + N] = r;
-                }
+                for (int d = 0; d < 3; ++d) sm.Rf[d][tid] = (float)(sm.R[d][tid] - sm.R[d][0]);
             __syncthreads();
         }
     }
@@ -720,12 +735,18 @@ struct Chain {
     __device__ __forceinline__ void apply(const double v[3], double y[3]) {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
         __syncthreads();
-        if constexpr (MIXED) {
+        if constexpr (S::PAIRS) {
+            if (own) {
+                const float vx = (float)v[0];
+                const f32x2 vyz = {(float)v[1], (float)v[2]};
+                sm.Pa[tid].w = vx;
+                sm.Pa[tid + N].w = vx;
+                sm.Pb[tid] = vyz;
+                sm.Pb[tid + N] = vyz;
+            }
+        } else if constexpr (MIXED) {
             if (own)
-                for (int d = 0; d < 3; ++d) {
-                    sm.W[d][tid] = (float)v[d];
-                    if constexpr (S::PAIRS) sm.W[d][tid + N] = (float)v[d];
-                }
+                for (int d = 0; d < 3; ++d) sm.W[d][tid] = (float)v[d];
         } else {
             for (int d = 0; d < 3; ++d) sm.X[d][tid] = v[d];
         }
@@ -832,16 +853,16 @@ struct Chain {
     __device__ __forceinline__ void apply_pairs(const double v[3], double y[3]) {
         RpyW<float> W;
         W.init(P.rpy_f);
-        // one base pointer for the six arrays: positions at +0, +FB, +2FB and
-        // the operand at +wo, +wo + 2 BLOCK, +wo + 4 BLOCK (floats), so each
-        // step needs one address and six immediate offsets
-        constexpr int FB = S::FB;
-        const lds_cptr<float> base = as_lds(&sm.Rf[0][0]);
-        const int wo = (int)(&sm.W[0][0] - &sm.Rf[0][0]);
+        // partner k of this lane: one 16-byte read (position, operand x) and
+        // one 8-byte read (operand y, z), both at immediate offsets from the
+        // lane's own entries
         const int me = own ? tid : 0;
-        const lds_cptr<float> mine = base + me;
-        const float rx = mine[0], ry = mine[FB], rz = mine[2 * FB];
-        const float xi = mine[wo], yi = mine[wo + 2 * BLOCK], zi = mine[wo + 4 * BLOCK];
+        const lds_cptr<f32x4> pa = as_lds(&sm.Pa[0]) + me;
+        const lds_cptr<f32x2> pb = as_lds(&sm.Pb[0]) + me;
+        const f32x4 mine = pa[0];
+        const f32x2 mine2 = pb[0];
+        const float rx = mine.x, ry = mine.y, rz = mine.z;
+        const float xi = mine.w, yi = mine2.x, zi = mine2.y;
         const int K = (N - 1) / 2;
         // lane j takes the moving accumulator from lane j - 1 (mod N)
         const int rot = own ? 4 * (tid == 0 ? N - 1 : tid - 1) : 4 * tid;
@@ -852,14 +873,15 @@ struct Chain {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         // own-row term of the pair (mu, mu + k); with `both` also the partner's
         auto pair = [&](int k, bool both, float& fx, float& fy, float& fz) {
-            const lds_cptr<float> q = mine + k;
-            const float bx = q[0] - rx, by = q[FB] - ry, bz = q[2 * FB] - rz;
+            const f32x4 q = pa[k];
+            const f32x2 q2 = pb[k];
+            const float bx = q.x - rx, by = q.y - ry, bz = q.z - rz;
             // the walk never meets the self pair: 1e-12 + |b|^2 stands for the
             // reference's clamp max(|b|^2, 1e-12) (the same value to FP32
             // resolution wherever |b|^2 > 1e-5)
             float om1, om2;
             W.weights_floor(fmaf(bz, bz, fmaf(by, by, fmaf(bx, bx, W.clamp))), om1, om2);
-            const float xx = q[wo], xy = q[wo + 2 * BLOCK], xz = q[wo + 4 * BLOCK];
+            const float xx = q.w, xy = q2.x, xz = q2.y;
             const float t = om2 * (bx * xx + by * xy + bz * xz);
             fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
             fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);

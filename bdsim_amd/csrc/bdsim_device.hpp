@@ -528,11 +528,11 @@ __host__ __device__ inline void rpy_folded(double hstar, double k[6]) {
 
 template <typename T>
 struct RpyW {
-    T K1, K2, K3, K4, K5, far2, clamp, cdiag;
+    T K1, K12, K13, K4, K5, far2, clamp, cdiag;
 
     // k: the six rpy_folded constants in the arithmetic type
     __device__ __forceinline__ void init(const T* k) {
-        K1 = k[0]; K2 = k[1]; K3 = k[2]; K4 = k[3]; K5 = k[4]; far2 = k[5];
+        K1 = k[0]; K12 = k[0] * k[1]; K13 = k[0] * k[2]; K4 = k[3]; K5 = k[4]; far2 = k[5];
         clamp = (T)1.0e-12;
         T om1, om2;
         weights((T)0, om1, om2);
@@ -555,13 +555,15 @@ struct RpyW {
         weights_floor(fmax(sq, clamp), om1, om2);
     }
 
-    // sq already at least the clamp (a separation formed as 1e-12 + |b|^2)
+    // sq already at least the clamp (a separation formed as 1e-12 + |b|^2).
+    // Far field as polynomials in 1/|b|: om1 = K1 ir + K1 K2 ir^3,
+    // om2 = ir^3 (K1 + K1 K3 ir^2).
     __device__ __forceinline__ void weights_floor(T sq, T& om1, T& om2) const {
         const T ir = rsq(sq);
         const T ir2 = ir * ir;
-        const T p = K1 * ir;
-        const T om1f = p * fma(K2, ir2, (T)1);
-        const T om2f = (p * ir2) * fma(K3, ir2, (T)1);
+        const T ir3 = ir * ir2;
+        const T om1f = fma(K12, ir3, K1 * ir);
+        const T om2f = ir3 * fma(K13, ir2, K1);
         const T om1n = fma(K4, sq * ir, (T)1);
         const T om2n = K5 * ir;
         const bool far = sq >= far2;
