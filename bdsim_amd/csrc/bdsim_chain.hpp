@@ -63,8 +63,11 @@ struct Smem {
     static constexpr int NW = BLOCK / 64;
     // single-wave chains walk bead pairs (mu, mu + k) on doubled FP32 copies
     static constexpr bool PAIRS = MIXED && BLOCK == 64;
-    static constexpr int FB = (MIXED && !PAIRS) ? BLOCK : 1;
-    static constexpr int PB = PAIRS ? 2 * BLOCK : 1;
+    // mixed form, 2..8 waves: the pair walk by 64-bead tiles (apply_tiles)
+    static constexpr bool TILES = MIXED && BLOCK >= 128 && BLOCK <= 512;
+    static constexpr int FB = (MIXED && !PAIRS && !TILES) ? BLOCK : 1;
+    static constexpr int PB = (PAIRS || TILES) ? 2 * BLOCK : 1;
+    static constexpr int AB = TILES ? BLOCK : 1;
     static constexpr int DB = MIXED ? 1 : BLOCK;   // FP64 operand (FP64 form)
     double R[3][BLOCK];         // configuration the mobility is evaluated at
     double P[3][BLOCK];         // force configuration; corrector positions
@@ -73,9 +76,16 @@ struct Smem {
     // mixed form, one wave (the pair walk): bead mu's FP32 position relative
     // to bead 0 and the operand's x in Pa[mu], the operand's y, z in Pb[mu],
     // both doubled (entry mu + N repeats mu), so that one 16-byte and one
-    // 8-byte read at an immediate offset fetch a partner
+    // 8-byte read at an immediate offset fetch a partner. Tiled walk: bead
+    // 64 t + i at 128 t + i and 128 t + 64 + i (pads beyond N far away, with
+    // a zero operand).
     f32x4 Pa[PB];
-    double B[4][BLOCK];         // bond-level arrays (unit bonds + lengths, bonds)
+    union {
+        double B[4][BLOCK];     // bond-level arrays (unit bonds + lengths, bonds)
+        // tiled walk: partner-row sums of other waves, fixed point (2^-30),
+        // added as integers so that the order of the waves does not matter
+        long long Acc[3][AB];
+    };
     union {
         double V[3][BLOCK];     // neighbour exchange; triangular-product operand
         float W[3][2 * BLOCK];  // mixed form: mat-vec operand, EV positions
@@ -626,7 +636,7 @@ struct Chain {
     bool sync_fail = false;     // SPLIT: a barrier timed out
     bool dense_ws = false;      // this step's factor went to the workspace
 
-    __device__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
+    __device__ __forceinline__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
                      FT* dense_blk = nullptr, double* dense_xs = nullptr)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
@@ -656,7 +666,21 @@ struct Chain {
         __syncthreads();
         for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
         __syncthreads();
-        if constexpr (S::PAIRS) {
+        if constexpr (S::TILES) {
+            const int e = 128 * (tid >> 6) + (tid & 63);
+            f32x4 r;
+            if (own) {
+                r.x = (float)(sm.R[0][tid] - sm.R[0][0]);
+                r.y = (float)(sm.R[1][tid] - sm.R[1][0]);
+                r.z = (float)(sm.R[2][tid] - sm.R[2][0]);
+            } else {
+                // a pad: far from the chain and from the other pads
+                r.x = 1.0e5f + 64.0f * (float)(tid & 63); r.y = 1.0e5f; r.z = 1.0e5f;
+            }
+            sm.Pa[e].x = r.x; sm.Pa[e].y = r.y; sm.Pa[e].z = r.z;
+            sm.Pa[e + 64].x = r.x; sm.Pa[e + 64].y = r.y; sm.Pa[e + 64].z = r.z;
+            __syncthreads();
+        } else if constexpr (S::PAIRS) {
             if (own) {
                 const float rx = (float)(sm.R[0][tid] - sm.R[0][0]);
                 const float ry = (float)(sm.R[1][tid] - sm.R[1][0]);
@@ -735,7 +759,16 @@ struct Chain {
     __device__ __forceinline__ void apply(const double v[3], double y[3]) {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
         __syncthreads();
-        if constexpr (S::PAIRS) {
+        if constexpr (S::TILES) {
+            const int e = 128 * (tid >> 6) + (tid & 63);
+            const float vx = own ? (float)v[0] : 0.0f;
+            const f32x2 vyz = {own ? (float)v[1] : 0.0f, own ? (float)v[2] : 0.0f};
+            sm.Pa[e].w = vx;
+            sm.Pa[e + 64].w = vx;
+            sm.Pb[e] = vyz;
+            sm.Pb[e + 64] = vyz;
+            for (int d = 0; d < 3; ++d) sm.Acc[d][tid] = 0;
+        } else if constexpr (S::PAIRS) {
             if (own) {
                 const float vx = (float)v[0];
                 const f32x2 vyz = {(float)v[1], (float)v[2]};
@@ -754,6 +787,10 @@ struct Chain {
         bump(sm.cnt.matvecs);
         if constexpr (S::PAIRS) {
             apply_pairs(v, y);
+            return;
+        }
+        if constexpr (S::TILES) {
+            apply_tiles(v, y);
             return;
         }
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -924,6 +961,119 @@ struct Chain {
         }
         if (own) { y[0] = v[0] + ax; y[1] = v[1] + ay; y[2] = v[2] + az; }
         else { y[0] = y[1] = y[2] = 0.0; }
+    }
+
+    // D x for a chain of 2..8 waves in the mixed form, each unordered bead
+    // pair evaluated once. The beads form tiles of 64 (one per wave, the last
+    // padded). Wave w walks (1) its own tile as the one-wave pair walk does
+    // (a ring of 64: offsets 31..1 with the partner-row terms on a register
+    // accumulator rotated one lane per step, and offset 32 from both ends,
+    // own-row terms only), and (2) every other tile t over 32 of the 64 lane
+    // offsets k (partner (l + k) mod 64): k = 63..32 if w < t, else 64..33,
+    // which between the two waves covers each cross pair once. The partner
+    // rows of (2) belong to wave t: their rotated sums go, every 16 steps, to
+    // Acc as fixed-point integers (exact and order-independent sums), read
+    // back after a barrier. FP32 runs of at most 16 steps are folded into
+    // FP64 sums. y = x + the sum (identity diagonal blocks).
+    __device__ __forceinline__ void apply_tiles(const double v[3], double y[3]) {
+        constexpr float FIX = 1073741824.0f;            // 2^30
+        RpyW<float> W;
+        W.init(P.rpy_f);
+        const int w = tid >> 6, l = tid & 63;
+        const int T = (N + 63) >> 6;
+        const int rot = 4 * ((l + 63) & 63);            // take from lane l - 1
+        auto perm = [](int addr, float x) {
+            return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+        };
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (w < T) {
+            const lds_cptr<f32x4> pa0 = as_lds(&sm.Pa[0]);
+            const lds_cptr<f32x2> pb0 = as_lds(&sm.Pb[0]);
+            const f32x4 mine = pa0[128 * w + l];
+            const f32x2 mine2 = pb0[128 * w + l];
+            const float rx = mine.x, ry = mine.y, rz = mine.z;
+            const float xi = mine.w, yi = mine2.x, zi = mine2.y;
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            lds_cptr<f32x4> pa = pa0 + 128 * w + l;
+            lds_cptr<f32x2> pb = pb0 + 128 * w + l;
+            auto pair = [&](int k, bool both, float& fx, float& fy, float& fz) {
+                const f32x4 q = pa[k];
+                const f32x2 q2 = pb[k];
+                const float bx = q.x - rx, by = q.y - ry, bz = q.z - rz;
+                float om1, om2;
+                W.weights_floor(fmaf(bz, bz, fmaf(by, by, fmaf(bx, bx, W.clamp))), om1, om2);
+                const float xx = q.w, xy = q2.x, xz = q2.y;
+                const float t = om2 * (bx * xx + by * xy + bz * xz);
+                fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
+                fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);
+                fz = fmaf(om1, xz, fz); fz = fmaf(t, bz, fz);
+                if (both) {
+                    const float u = om2 * (bx * xi + by * yi + bz * zi);
+                    sx = fmaf(om1, xi, sx); sx = fmaf(u, bx, sx);
+                    sy = fmaf(om1, yi, sy); sy = fmaf(u, by, sy);
+                    sz = fmaf(om1, zi, sz); sz = fmaf(u, bz, sz);
+                    sx = perm(rot, sx); sy = perm(rot, sy); sz = perm(rot, sz);
+                }
+            };
+            // (1) own tile: after step kend the rotated terms are for row
+            // l + kend - 1 ... brought to their lane by one permute
+            {
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                pair(32, false, fx, fy, fz);
+                ax += (double)fx; ay += (double)fy; az += (double)fz;
+                auto flush = [&](int kend, float gx, float gy, float gz) {
+                    ax += (double)gx; ay += (double)gy; az += (double)gz;
+                    const int from = 4 * ((l - kend) & 63);
+                    ax += (double)perm(from, sx);
+                    ay += (double)perm(from, sy);
+                    az += (double)perm(from, sz);
+                    sx = sy = sz = 0.0f;
+                };
+                int k = 31;
+                for (; k >= 16; k -= 16) {
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll 4
+                    for (int u = 0; u < 16; ++u) pair(k - u, true, gx, gy, gz);
+                    flush(k - 16, gx, gy, gz);
+                }
+                {
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                    for (; k > 0; --k) pair(k, true, gx, gy, gz);
+                    flush(0, gx, gy, gz);
+                }
+            }
+            // (2) the other tiles: 32 offsets each, in two runs of 16; after a
+            // run ending at offset kend, lane j's rotated sums are for row
+            // (j - 1 + kend) mod 64 of tile t
+            for (int t = 0; t < T; ++t) {
+                if (t == w) continue;
+                pa = pa0 + 128 * t + l;
+                pb = pb0 + 128 * t + l;
+                const int khi = w < t ? 63 : 64;
+#pragma unroll 1
+                for (int r = 0; r < 2; ++r) {
+                    const int k0 = khi - 16 * r;
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll 4
+                    for (int u = 0; u < 16; ++u) pair(k0 - u, true, gx, gy, gz);
+                    ax += (double)gx; ay += (double)gy; az += (double)gz;
+                    const int row = 64 * t + ((l + k0 - 16) & 63);
+                    atomicAdd((unsigned long long*)&sm.Acc[0][row], (unsigned long long)(long long)__float2ll_rn(sx * FIX));
+                    atomicAdd((unsigned long long*)&sm.Acc[1][row], (unsigned long long)(long long)__float2ll_rn(sy * FIX));
+                    atomicAdd((unsigned long long*)&sm.Acc[2][row], (unsigned long long)(long long)__float2ll_rn(sz * FIX));
+                    sx = sy = sz = 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+        if (own) {
+            const double inv = 1.0 / 1073741824.0;
+            y[0] = v[0] + (ax + (double)sm.Acc[0][tid] * inv);
+            y[1] = v[1] + (ay + (double)sm.Acc[1][tid] * inv);
+            y[2] = v[2] + (az + (double)sm.Acc[2][tid] * inv);
+        } else {
+            y[0] = y[1] = y[2] = 0.0;
+        }
     }
 
     // Spring force on the beads from the bonds held by threads tid < N-1
