@@ -82,8 +82,9 @@ struct Smem {
     f32x4 Pa[PB];
     union {
         double B[4][BLOCK];     // bond-level arrays (unit bonds + lengths, bonds)
-        // tiled walk: partner-row sums of other waves, fixed point (2^-30),
-        // added as integers so that the order of the waves does not matter
+        // tiled walk: partner-row sums of other waves in fixed point (the
+        // scale set by the operand's largest entry), added as integers so
+        // that the order of the waves does not matter
         long long Acc[3][AB];
     };
     union {
@@ -96,6 +97,7 @@ struct Smem {
     double Beta[LANCZOS_COLD];
     double Scal[16];
     int32_t ScalI[8];
+    int32_t Ex[TILES ? NW : 1]; // tiled walk: per wave, the largest biased exponent of the operand
     Counters cnt;               // work counters, updated by thread 0
     StepLog lg;                 // this step's decisions, written by thread 0
 };
@@ -768,6 +770,12 @@ struct Chain {
             sm.Pb[e] = vyz;
             sm.Pb[e + 64] = vyz;
             for (int d = 0; d < 3; ++d) sm.Acc[d][tid] = 0;
+            // the biased exponent of the largest |entry| (bit patterns of
+            // non-negative floats order as integers)
+            int eb = __float_as_int(fmaxf(fabsf(vx), fmaxf(fabsf(vyz.x), fabsf(vyz.y)))) >> 23;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) eb = max(eb, __shfl_xor(eb, o, 64));
+            if ((tid & 63) == 0) sm.Ex[tid >> 6] = eb;
         } else if constexpr (S::PAIRS) {
             if (own) {
                 const float vx = (float)v[0];
@@ -973,10 +981,21 @@ struct Chain {
     // which between the two waves covers each cross pair once. The partner
     // rows of (2) belong to wave t: their rotated sums go, every 16 steps, to
     // Acc as fixed-point integers (exact and order-independent sums), read
-    // back after a barrier. FP32 runs of at most 16 steps are folded into
-    // FP64 sums. y = x + the sum (identity diagonal blocks).
+    // back after a barrier. The fixed-point unit is 2^-ks with ks = 170 - e,
+    // e the largest biased FP32 exponent of the operand's entries (|x| <
+    // 2^(e-126)): a pair term's components are below 2 |x|max (RPY weights:
+    // om1 <= 1, om2 |b|^2 < 0.21), so a row's sum over N <= 512 partners stays
+    // below 2^(1+9+e-126) units of 2^-ks, i.e. 2^54, and the unit is at most
+    // 2^-43 of the largest entry, far below the FP32 rounding of the runs it
+    // adds. FP32 runs of at most 16 steps are folded into FP64 sums.
+    // y = x + the sum (identity diagonal blocks).
     __device__ __forceinline__ void apply_tiles(const double v[3], double y[3]) {
-        constexpr float FIX = 1073741824.0f;            // 2^30
+        int e = 0;
+#pragma unroll
+        for (int i = 0; i < S::NW; ++i) e = max(e, sm.Ex[i]);
+        const int ks = min(170 - e, 126);                // in [-85, 126]
+        const float FIX = __int_as_float((127 + ks) << 23);         // 2^ks
+        const double inv = __hiloint2double((1023 - ks) << 20, 0); // 2^-ks
         RpyW<float> W;
         W.init(P.rpy_f);
         const int w = tid >> 6, l = tid & 63;
@@ -1067,7 +1086,6 @@ struct Chain {
         }
         __syncthreads();
         if (own) {
-            const double inv = 1.0 / 1073741824.0;
             y[0] = v[0] + (ax + (double)sm.Acc[0][tid] * inv);
             y[1] = v[1] + (ay + (double)sm.Acc[1][tid] * inv);
             y[2] = v[2] + (az + (double)sm.Acc[2][tid] * inv);
