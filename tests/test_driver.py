@@ -216,20 +216,3 @@ def test_parameter_pickle_tuples_match_reference_layout():
     r = bd.Rng(7)
     assert r.__getstate__() == (1, -1, 7)
     assert pickle.loads(pickle.dumps(r)).__getstate__() == (1, -1, 7)
-
-
-def test_fene_fraenkel_sampler_distribution():
-    """Support [max(0, s-dQ), s+dQ] and a KS test against
-    psi(Q) ~ Q^2 (1-(Q-s)^2/dQ^2)^(dQ^2/2) (python/bdsim/initial.py:29-41)."""
-    from scipy import integrate, stats
-    s, dQ = 1.0, 2.0
-    R = np.array([bd.fene_fraenkel_chain(201, s, dQ, seed=k) for k in range(20)])
-    Q = np.linalg.norm(np.diff(R, axis=1), axis=2).ravel()
-    assert Q.min() >= max(0.0, s - dQ) and Q.max() <= s + dQ
-    psi = lambda q: q * q * (1 - (q - s) ** 2 / dQ ** 2) ** (dQ ** 2 / 2)  # noqa: E731
-    Z = integrate.quad(psi, 0.0, s + dQ)[0]
-    grid = np.linspace(0.0, s + dQ, 2001)
-    cdf_grid = np.array([integrate.quad(psi, 0.0, x)[0] / Z for x in grid])
-    assert stats.kstest(Q, lambda x: np.interp(x, grid, cdf_grid)).pvalue > 1e-3
-    g = bd.gaussian_chain(50, seed=3)
-    assert np.allclose(g.mean(axis=0), 0.0, atol=1e-12)
