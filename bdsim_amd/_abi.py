@@ -234,6 +234,7 @@ def _declare_gpu(lib):
     lib.bdsim_batch_create.argtypes = [P(c_void_p), c_int, c_int, _dp, P(CPhys), _dp,
                                        P(CRng), c_int]
     lib.bdsim_batch_run.argtypes = [c_void_p, P(CSim)]
+    lib.bdsim_batch_run_steps.argtypes = [c_void_p, P(CSim), c_int, c_int]
     lib.bdsim_batch_sync.argtypes = [c_void_p]
     lib.bdsim_batch_fetch.argtypes = [c_void_p, _dp, P(CRng), _ip, P(CCounters)]
     lib.bdsim_batch_stream.argtypes = [c_void_p]

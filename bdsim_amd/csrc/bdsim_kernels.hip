@@ -14,6 +14,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 #include <cstdio>
 #include <cstring>
@@ -359,6 +360,13 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// Opens a resumable segment at t0 on n trajectories: cold spectral cache,
+// nothing done yet.
+__global__ void k_seg_open(SegState* seg, int n, double t0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) seg[i] = SegState{t0, 0, 0};
+}
+
 template <bool MIXED>
 int launch_split(int count, int G, hipStream_t stream, KArgs a, uint32_t* bar, double* xbuf) {
     hipLaunchKernelGGL(k_zero_u32, dim3((count + 255) / 256), dim3(256), 0, stream, bar, count);
@@ -597,6 +605,10 @@ struct bdsim_batch {
     void* obs = nullptr;            // observables records, allocated on first use
     uint32_t* bar = nullptr;        // split chains: barrier counters, exchange buffers
     double* xbuf = nullptr;
+    SegState* seg = nullptr;        // resumable runs (bdsim_batch_run_steps), on first use
+    int32_t* done = nullptr;
+    bdsim_sim seg_sim{};            // the segment those runs are cutting
+    bool seg_open = false;
 };
 
 extern "C" {
@@ -824,6 +836,7 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads, const double* 
 int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
     if (!b || !sim) return fail(BDSIM_EINVAL, "null argument");
     if (int rc = check_clock(sim->time_start, sim->time_end, sim->dt)) return rc;
+    b->seg_open = false;            // a whole segment ends any one being cut into runs
     KArgs a{};
     a.tp = b->tp;
     a.list = b->list;
@@ -855,6 +868,66 @@ int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
     return 0;
 }
 
+int bdsim_batch_run_steps(bdsim_batch* b, const bdsim_sim* sim, int max_steps, int resume) {
+    if (!b || !sim) return fail(BDSIM_EINVAL, "null argument");
+    if (max_steps <= 0) return fail(BDSIM_EINVAL, "max_steps must be positive");
+    if (b->split > 1) return fail(BDSIM_EINVAL, "resumable runs of split chains are not supported");
+    if (int rc = check_clock(sim->time_start, sim->time_end, sim->dt)) return rc;
+    if (resume) {
+        if (!b->seg_open) return fail(BDSIM_EINVAL, "no segment to resume");
+        const bdsim_sim& o = b->seg_sim;
+        if (sim->time_start != o.time_start || sim->time_end != o.time_end || sim->dt != o.dt ||
+            sim->implicit_loop_tol != o.implicit_loop_tol ||
+            (sim->update_center_of_mass != 0) != (o.update_center_of_mass != 0))
+            return fail(BDSIM_EINVAL, "resume with the segment's own bdsim_sim");
+    }
+    if (!b->seg &&
+        (hipMalloc(&b->seg, sizeof(SegState) * b->n_traj) != hipSuccess ||
+         hipMalloc(&b->done, sizeof(int32_t) * b->n_traj) != hipSuccess))
+        return fail(BDSIM_ENOMEM, "hipMalloc failed for the segment state");
+    const int n = b->n_traj;
+    if (!resume) {
+        // counters run on across the launches of one segment: start them at zero
+        static_assert(sizeof(Counters) % 4 == 0, "Counters is zeroed as 32-bit words");
+        const int words = (int)(sizeof(Counters) / 4) * n;
+        hipLaunchKernelGGL(k_zero_u32, dim3((words + 255) / 256), dim3(256), 0, b->stream,
+                           reinterpret_cast<uint32_t*>(b->cnt), words);
+        hipLaunchKernelGGL(k_seg_open, dim3((n + 255) / 256), dim3(256), 0, b->stream, b->seg, n,
+                           sim->time_start);
+        HIP_TRY(hipGetLastError());
+        b->seg_sim = *sim;
+        b->seg_open = true;
+    }
+    KArgs a{};
+    a.tp = b->tp;
+    a.list = b->list;
+    a.R = b->R;
+    a.ws = b->ws;
+    a.blob = b->blob;
+    a.ext = b->ext;
+    a.rng = b->rng;
+    a.status = b->status;
+    a.cnt = b->cnt;
+    a.log = nullptr;
+    a.jx = g.jx;
+    a.jy = g.jy;
+    a.sim_override = 1;
+    a.com = sim->update_center_of_mass ? 1 : 0;
+    a.t0 = sim->time_start;
+    a.t1 = sim->time_end;
+    a.dt = sim->dt;
+    a.tol = sim->implicit_loop_tol;
+    a.seg = b->seg;
+    a.done_out = b->done;
+    a.max_steps = max_steps;
+    if (is_mixed(b->precision))
+        launch_integrate<true>(b->block, dim3(n), b->stream, a, b->dense_nb, -1,
+                               b->precision == BDSIM_PRECISION_MIXED_F32_FACTOR);
+    else launch_integrate<false>(b->block, dim3(n), b->stream, a, b->dense_nb);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 int bdsim_batch_sync(bdsim_batch* b) {
     if (!b) return fail(BDSIM_EINVAL, "null batch");
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -881,6 +954,7 @@ void bdsim_batch_destroy(bdsim_batch* b) {
     (void)hipFree(b->blob); (void)hipFree(b->ext); (void)hipFree(b->rng);
     (void)hipFree(b->status); (void)hipFree(b->cnt); (void)hipFree(b->obs);
     (void)hipFree(b->bar); (void)hipFree(b->xbuf);
+    (void)hipFree(b->seg); (void)hipFree(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -12,12 +12,16 @@ validation/hi_nan_probe.py:103).
 Chain i uses Rng(seed=i) and a Gaussian random-walk start (bond std 1) drawn
 from numpy's generator with seed i. Data is synthetic.
 
-A "step" is one integrator step of every chain. W warm-up steps run as one
-integrate segment, then K timed steps run as ONE further segment (one cold
-Lanczos start, as a single integrate call does), with positions, RNG states and
-workspace resident in HBM. `value` is the mixed-precision form's
+A "step" is one integrator step of every chain. Each chain integrates ONE
+segment of W + K steps (one integrate call: a single cold Lanczos start, at its
+first step, as the reference's run_ensemble integrates a trajectory), with
+positions, RNG states and workspace resident in HBM; the first W steps run
+untimed, then the K timed steps continue the segment in one launch
+(bdsim_batch_run_steps). `value` is the mixed-precision form's
 bead-steps/s = 4096 * 50 * K / seconds; the plain FP64 form is timed the same
-way and reported beside it.
+way and reported beside it. `fresh_segment` times the same K steps as a segment
+of their own (a separate integrate call, its cold start inside the timed
+region), the cost of short segments.
 
 roofline: algorithmic FLOPs of one launch = sum over chains of 18 N^2 per
 mobility product (a dense 3N x 3N mat-vec) times the mat-vecs the run itself
@@ -104,7 +108,23 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
 
 
-def run_gpu(lib, precision, steps, warmup, R0, K, ev):
+def _counter_arrays(lib, b, n_traj):
+    cnt = (A.CCounters * n_traj)()
+    A.check(lib, lib.bdsim_batch_fetch(b, None, None, None, cnt))
+    return {k: np.array([getattr(c, k) for c in cnt], dtype=np.float64)
+            for k in ("matvecs", "steps", "lanczos_cold", "fallbacks", "corrector_iters")}
+
+
+def run_gpu(lib, precision, steps, warmup, R0, K, ev, fresh=False):
+    """Time `steps` steps of every chain on a resident batch.
+
+    Default: the chains integrate ONE segment of warmup + steps steps (what a
+    long run_ensemble integration is: one integrate call per trajectory, one
+    cold Lanczos start); its first `warmup` steps run untimed, the timed
+    launch continues the segment (bdsim_batch_run_steps resumes it from the
+    stored time, warm seed and generator state). fresh=True: the timed steps
+    are a segment of their own after a separate warm-up segment, so the timed
+    region includes a segment's cold start."""
     phys = headline_phys()
     n_traj = R0.shape[0]
     rng = (A.CRng * n_traj)(*[A.CRng(1, -1, i) for i in range(n_traj)])
@@ -113,37 +133,79 @@ def run_gpu(lib, precision, steps, warmup, R0, K, ev):
                                         A.dptr(K), rng, precision))
     try:
         dt = 1e-3
-        if warmup > 0:
-            sim = A.CSim(0.0, (warmup - 1) * dt, dt, 1e-3, 1)
-            A.check(lib, lib.bdsim_batch_run(b, byref(sim)))
-            A.check(lib, lib.bdsim_batch_sync(b))
         stream = lib.bdsim_batch_stream(b)
         e0, e1 = ev.create(), ev.create()
-        t_start = warmup * dt
-        sim = A.CSim(t_start, t_start + (steps - 1) * dt, dt, 1e-3, 1)
+        if fresh:
+            if warmup > 0:
+                sim = A.CSim(0.0, (warmup - 1) * dt, dt, 1e-3, 1)
+                A.check(lib, lib.bdsim_batch_run(b, byref(sim)))
+            t_start = warmup * dt
+            sim = A.CSim(t_start, t_start + (steps - 1) * dt, dt, 1e-3, 1)
+            launch = lambda: lib.bdsim_batch_run(b, byref(sim))  # noqa: E731
+            before = None
+        else:
+            sim = A.CSim(0.0, (warmup + steps - 1) * dt, dt, 1e-3, 1)
+            if warmup > 0:
+                A.check(lib, lib.bdsim_batch_run_steps(b, byref(sim), warmup, 0))
+            launch = lambda: lib.bdsim_batch_run_steps(b, byref(sim), steps, 1 if warmup > 0 else 0)  # noqa: E731
         A.check(lib, lib.bdsim_batch_sync(b))
+        if not fresh:
+            before = _counter_arrays(lib, b, n_traj) if warmup > 0 else None
         w0 = time.perf_counter()
         ev.record(e0, stream)
-        A.check(lib, lib.bdsim_batch_run(b, byref(sim)))
+        A.check(lib, launch())
         ev.record(e1, stream)
         A.check(lib, lib.bdsim_batch_sync(b))
         wall = time.perf_counter() - w0
         kernel_ms = ev.elapsed_ms(e0, e1)
-        cnt = (A.CCounters * n_traj)()
+        after = _counter_arrays(lib, b, n_traj)
         status = np.zeros(n_traj, dtype=np.int32)
         Rout = np.empty_like(R0)
-        A.check(lib, lib.bdsim_batch_fetch(b, A.dptr(Rout), None, A.iptr(status), cnt))
+        A.check(lib, lib.bdsim_batch_fetch(b, A.dptr(Rout), None, A.iptr(status), None))
     finally:
         lib.bdsim_batch_destroy(b)
-    mv = np.array([c.matvecs for c in cnt], dtype=np.float64)
-    st = np.array([c.steps for c in cnt])
-    flops = float(np.sum(18.0 * N_BEADS * N_BEADS * mv))
+    c = {k: after[k] - (before[k] if before is not None else 0.0) for k in after}
+    flops = float(np.sum(18.0 * N_BEADS * N_BEADS * c["matvecs"]))
+    n_steps = max(c["steps"].sum(), 1.0)
     return dict(wall_s=wall, kernel_ms=kernel_ms, flops=flops, status_ok=int(np.sum(status == 0)),
-                steps_done=int(st.min()), matvecs_per_step=float(mv.sum() / max(st.sum(), 1)),
-                finite=bool(np.all(np.isfinite(Rout))),
-                lanczos_cold=int(sum(c.lanczos_cold for c in cnt)),
-                fallbacks=int(sum(c.fallbacks for c in cnt)),
-                corrector_per_step=float(sum(c.corrector_iters for c in cnt) / max(st.sum(), 1)))
+                steps_done=int(c["steps"].min()), matvecs_per_step=float(c["matvecs"].sum() / n_steps),
+                finite=bool(np.all(np.isfinite(Rout))), lanczos_cold=int(c["lanczos_cold"].sum()),
+                fallbacks=int(c["fallbacks"].sum()),
+                corrector_per_step=float(c["corrector_iters"].sum() / n_steps))
+
+
+def per_step_launches(lib, precision, n_steps, R0, K, ev):
+    """The first n_steps steps of one segment of the headline batch as one
+    launch per step (resumed segment): per step, the launch time (HIP events)
+    and the mean and largest number of mat-vecs over the chains. Shows where
+    a short segment's time goes: the cold first step, and the early steps
+    from the random-walk start, whose launches last as long as their costliest
+    chains."""
+    phys = headline_phys()
+    n_traj = R0.shape[0]
+    rng = (A.CRng * n_traj)(*[A.CRng(1, -1, i) for i in range(n_traj)])
+    b = ctypes.c_void_p()
+    A.check(lib, lib.bdsim_batch_create(byref(b), n_traj, N_BEADS, A.dptr(R0), byref(phys),
+                                        A.dptr(K), rng, precision))
+    out = []
+    try:
+        sim = A.CSim(0.0, (n_steps - 1) * 1e-3, 1e-3, 1e-3, 1)
+        stream = lib.bdsim_batch_stream(b)
+        prev = None
+        for k in range(n_steps):
+            e0, e1 = ev.create(), ev.create()
+            ev.record(e0, stream)
+            A.check(lib, lib.bdsim_batch_run_steps(b, byref(sim), 1, 1 if k else 0))
+            ev.record(e1, stream)
+            A.check(lib, lib.bdsim_batch_sync(b))
+            c = _counter_arrays(lib, b, n_traj)["matvecs"]
+            mv = c - (prev if prev is not None else 0.0)
+            prev = c
+            out.append({"step": k, "ms": round(ev.elapsed_ms(e0, e1), 4),
+                        "matvecs_mean": round(float(mv.mean()), 3), "matvecs_max": int(mv.max())})
+    finally:
+        lib.bdsim_batch_destroy(b)
+    return out
 
 
 def run_workload(lib, name, precision, steps, warmup):
@@ -464,6 +526,9 @@ def main():
     ap.add_argument("--serving-step-scale", type=float, default=1.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--per-step-launches", type=int, default=0, metavar="K",
+                    help="only time the first K steps of a headline segment, one launch per "
+                         "step, and print one line per form")
     ap.add_argument("--workload", choices=["headline", "uneven", "long", "cholesky", "c1"],
                     default="headline",
                     help="uneven: 4096 chains with N log-uniform in [10,256]; "
@@ -503,6 +568,12 @@ def main():
     ev = HipEvents()
     R0 = start_configs(args.traj, N_BEADS)
     K = kappas(args.traj)
+    if args.per_step_launches:
+        for form in args.forms.split(","):
+            print(f"bench: {form} form, one launch per step", file=sys.stderr, flush=True)
+            print(json.dumps({"form": form, "per_step": per_step_launches(
+                lib, PRECISION[form], args.per_step_launches, R0, K, ev)}), flush=True)
+        return
     res = {}
     for form in args.forms.split(","):
         prec = PRECISION[form]
@@ -510,6 +581,15 @@ def main():
               file=sys.stderr, flush=True)
         res[form] = run_gpu(lib, prec, args.steps, args.warmup, R0, K, ev)
         print(f"bench: {form}: {json.dumps(res[form])}", file=sys.stderr, flush=True)
+        assert res[form]["steps_done"] == args.steps, res[form]
+    fresh = {}
+    for form in args.forms.split(","):
+        print(f"bench: {form} form, a fresh {args.steps}-step segment", file=sys.stderr, flush=True)
+        f = run_gpu(lib, PRECISION[form], args.steps, args.warmup, R0, K, ev, fresh=True)
+        fresh[form] = {"value": args.traj * N_BEADS * args.steps / (f["kernel_ms"] / 1e3),
+                       "ms_per_step": f["kernel_ms"] / args.steps,
+                       "matvecs_per_step": f["matvecs_per_step"],
+                       "lanczos_cold": f["lanczos_cold"]}
     main_form = "mixed" if "mixed" in res else "fp64"
     r = res[main_form]
     bead_steps = args.traj * N_BEADS * args.steps
@@ -531,7 +611,10 @@ def main():
         "dtype": "f32+f64" if main_form == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.traj} chains x N=50 FENE b=100, RPY h*=0.3 Chebyshev, "
-                               "SDK EV, shear Wi {0.1,1,10,100}/tau_1 round robin, dt=1e-3, one segment",
+                               "SDK EV, shear Wi {0.1,1,10,100}/tau_1 round robin, dt=1e-3",
+                   "segment": (f"each chain integrates one segment of {args.warmup}+{args.steps} "
+                               f"steps (one integrate call, cold Lanczos start in the warm-up); "
+                               f"the {args.steps} timed steps continue it"),
                    "form": main_form, "n_traj": args.traj, "n_beads": N_BEADS},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[main_form],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_form],
@@ -550,6 +633,11 @@ def main():
                        "achieved_tflops": f["flops"] / (f["kernel_ms"] / 1e3) / 1e12,
                        "frac_of_78.6": f["flops"] / (f["kernel_ms"] / 1e3) / 1e12 / 78.6}
         out["mixed_over_fp64"] = out["value"] / out["fp64"]["value"]
+    out["fresh_segment"] = dict(fresh, note=(f"the {args.steps} timed steps as a segment of their "
+                                             "own (a separate integrate call after the warm-up): "
+                                             "the timed region includes that segment's cold start"))
+    if "mixed" in fresh and "fp64" in fresh:
+        out["fresh_segment"]["mixed_over_fp64"] = fresh["mixed"]["value"] / fresh["fp64"]["value"]
     if "fp64" in res and "mixed" in res:
         print("bench: mixed-form errors", file=sys.stderr, flush=True)
         out["mixed_errors"] = mixed_errors(lib)

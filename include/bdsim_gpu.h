@@ -256,6 +256,16 @@ int bdsim_batch_create(bdsim_batch** out, int n_traj, int nbeads,
 /* Runs one integrate segment [sim->time_start, sim->time_end] on every
  * trajectory of the batch on the batch's stream; does not synchronise.     */
 int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim);
+/* Runs the next max_steps steps at most of the segment [sim->time_start,
+ * sim->time_end] on every trajectory, on the batch's stream (no
+ * synchronisation). resume = 0 starts the segment (cold spectral cache,
+ * counters from zero); resume = 1, with the same *sim, continues each
+ * trajectory where the previous call stopped (its time, warm Lanczos seed and
+ * generator state), so a segment cut into calls gives, bit for bit, the
+ * result of one bdsim_batch_run over it (what one time_integrate_chain call,
+ * src/integrator.hpp:32-33, does for each chain). Not for batches of split
+ * chains (N > 512 spread over several workgroups): BDSIM_EINVAL.            */
+int bdsim_batch_run_steps(bdsim_batch* b, const bdsim_sim* sim, int max_steps, int resume);
 /* Waits for the batch's stream. */
 int bdsim_batch_sync(bdsim_batch* b);
 /* Copies positions / rng / status / counters back (any may be NULL). */

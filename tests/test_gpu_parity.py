@@ -390,6 +390,76 @@ def test_resident_split_chains_match(gpu):
         assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st[i].ix, st[i].iy, st[i].seed)
 
 
+def _batch_segment(gpu, R0, p, prec, sim, cuts):
+    """One segment on a resident batch: whole (cuts None) or cut into runs of
+    the given step counts (the first starts it, the others resume)."""
+    n, N = R0.shape[0], R0.shape[1]
+    rng = (A.CRng * n)(*[H.rng_of(10 + i) for i in range(n)])
+    b = ctypes.c_void_p()
+    A.check(gpu, gpu.bdsim_batch_create(byref(b), n, N, A.dptr(R0), byref(p.c), None, rng, prec))
+    try:
+        if cuts is None:
+            A.check(gpu, gpu.bdsim_batch_run(b, byref(sim)))
+        else:
+            for k, m in enumerate(cuts):
+                A.check(gpu, gpu.bdsim_batch_run_steps(b, byref(sim), m, 1 if k else 0))
+        R = np.empty_like(R0)
+        st = (A.CRng * n)()
+        status = np.zeros(n, dtype=np.int32)
+        cnt = (A.CCounters * n)()
+        A.check(gpu, gpu.bdsim_batch_fetch(b, A.dptr(R), st, A.iptr(status), cnt))
+    finally:
+        gpu.bdsim_batch_destroy(b)
+    return R, [(s.ix, s.iy, s.seed) for s in st], status, [bytes(c) for c in cnt]
+
+
+@pytest.mark.parametrize("kind", ["cheb64", "cheb130", "cholesky30"])
+def test_resident_segment_cut_into_runs(gpu, kind):
+    """bdsim_batch_run_steps: a 30-step segment cut into runs of 7, 13, 9 and
+    5 steps (the last finds 1 step left) equals one bdsim_batch_run over it
+    bit for bit: positions, generator states, status and counters, both forms
+    (the resumed runs start from the stored time and warm Lanczos seed)."""
+    N = {"cheb64": 50, "cheb130": 130, "cholesky30": 30}[kind]
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=3 if N == 50 else 1, zstar=0.5, dstar=1.0,
+                    method=1 if kind == "cholesky30" else 0, kappa=C.wi_kappa(3.0))
+    p = H.Packed(d, N)
+    R0 = np.ascontiguousarray(np.array([C.random_walk(N, 900 + i, 0.9) for i in range(5)]))
+    sim = A.CSim(0.0, 0.029, 1e-3, 1e-3, 1)
+    precs = [A.PRECISION_FP64, A.PRECISION_MIXED]
+    if kind == "cholesky30":
+        precs.append(A.PRECISION_MIXED_F32_FACTOR)
+    for prec in precs:
+        whole = _batch_segment(gpu, R0, p, prec, sim, None)
+        cut = _batch_segment(gpu, R0, p, prec, sim, [7, 13, 9, 5])
+        assert np.all(whole[2] == 0)
+        assert np.array_equal(whole[0], cut[0]), prec
+        assert whole[1] == cut[1] and np.array_equal(whole[2], cut[2]) and whole[3] == cut[3], prec
+
+
+def test_resident_resume_needs_its_segment(gpu):
+    """Resuming without a started segment, or with another clock, is refused."""
+    N, n = 20, 2
+    p = H.Packed(C.phys_dict(spring=2, sqrtb=10.0, hstar=0.2), N)
+    R0 = np.ascontiguousarray(np.array([C.random_walk(N, 5 + i) for i in range(n)]))
+    rng = (A.CRng * n)(*[H.rng_of(i) for i in range(n)])
+    b = ctypes.c_void_p()
+    A.check(gpu, gpu.bdsim_batch_create(byref(b), n, N, A.dptr(R0), byref(p.c), None, rng,
+                                        A.PRECISION_FP64))
+    try:
+        sim = A.CSim(0.0, 0.009, 1e-3, 1e-3, 1)
+        assert gpu.bdsim_batch_run_steps(b, byref(sim), 3, 1) == -1
+        assert gpu.bdsim_batch_run_steps(b, byref(sim), 0, 0) == -1
+        A.check(gpu, gpu.bdsim_batch_run_steps(b, byref(sim), 3, 0))
+        other = A.CSim(0.0, 0.019, 1e-3, 1e-3, 1)
+        assert gpu.bdsim_batch_run_steps(b, byref(other), 3, 1) == -1
+        A.check(gpu, gpu.bdsim_batch_run_steps(b, byref(sim), 3, 1))
+        A.check(gpu, gpu.bdsim_batch_run(b, byref(sim)))          # a whole segment closes it
+        assert gpu.bdsim_batch_run_steps(b, byref(sim), 3, 1) == -1
+        A.check(gpu, gpu.bdsim_batch_sync(b))
+    finally:
+        gpu.bdsim_batch_destroy(b)
+
+
 def test_mixed_precision_within_1e5(gpu, orc):
     """Headline physics (N=50, FENE b=100, h*=0.3 Chebyshev, SDK, shear):
     the mixed form stays within 1e-5 relative of the FP64 oracle over 100 steps."""
