@@ -63,8 +63,8 @@ struct Smem {
     static constexpr int NW = BLOCK / 64;
     // single-wave chains walk bead pairs (mu, mu + k) on doubled FP32 copies
     static constexpr bool PAIRS = MIXED && BLOCK == 64;
-    // mixed form, 2..8 waves: the pair walk by 64-bead tiles (apply_tiles)
-    static constexpr bool TILES = MIXED && BLOCK >= 128 && BLOCK <= 512;
+    // mixed form, 2..16 waves: the pair walk by 64-bead tiles (apply_tiles)
+    static constexpr bool TILES = MIXED && BLOCK >= 128 && BLOCK <= 1024;
     static constexpr int FB = (MIXED && !PAIRS && !TILES) ? BLOCK : 1;
     static constexpr int PB = (PAIRS || TILES) ? 2 * BLOCK : 1;
     static constexpr int AB = TILES ? BLOCK : 1;
@@ -799,6 +799,12 @@ struct Chain {
         }
         if constexpr (S::TILES) {
             apply_tiles(v, y);
+            if constexpr (SPLIT) {
+                // y holds this part's share of the pair terms; every thread
+                // takes part in the exchange
+                exchange(y[0], y[1], y[2]);
+                if (own) { y[0] += v[0]; y[1] += v[1]; y[2] += v[2]; }
+            }
             return;
         }
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -971,7 +977,7 @@ struct Chain {
         else { y[0] = y[1] = y[2] = 0.0; }
     }
 
-    // D x for a chain of 2..8 waves in the mixed form, each unordered bead
+    // D x for a chain of 2..16 waves in the mixed form, each unordered bead
     // pair evaluated once. The beads form tiles of 64 (one per wave, the last
     // padded). Wave w walks (1) its own tile as the one-wave pair walk does
     // (a ring of 64: offsets 31..1 with the partner-row terms on a register
@@ -984,11 +990,13 @@ struct Chain {
     // back after a barrier. The fixed-point unit is 2^-ks with ks = 170 - e,
     // e the largest biased FP32 exponent of the operand's entries (|x| <
     // 2^(e-126)): a pair term's components are below 2 |x|max (RPY weights:
-    // om1 <= 1, om2 |b|^2 < 0.21), so a row's sum over N <= 512 partners stays
-    // below 2^(1+9+e-126) units of 2^-ks, i.e. 2^54, and the unit is at most
-    // 2^-43 of the largest entry, far below the FP32 rounding of the runs it
-    // adds. FP32 runs of at most 16 steps are folded into FP64 sums.
-    // y = x + the sum (identity diagonal blocks).
+    // om1 <= 1, om2 |b|^2 < 0.21), so a row's sum over N <= 1024 partners
+    // stays below 2^(1+10+e-126) units of 2^-ks, i.e. 2^55, and the unit is at
+    // most 2^-43 of the largest entry, far below the FP32 rounding of the runs
+    // it adds. FP32 runs of at most 16 steps are folded into FP64 sums.
+    // y = x + the sum (identity diagonal blocks). A chain split over parts
+    // (SPLIT): part p walks the tile pairs (w, t) with t = p (mod parts), own
+    // tiles included, and y is its share of the sum alone (no x).
     __device__ __forceinline__ void apply_tiles(const double v[3], double y[3]) {
         int e = 0;
 #pragma unroll
@@ -1036,7 +1044,7 @@ struct Chain {
             };
             // (1) own tile: after step kend the rotated terms are for row
             // l + kend - 1 ... brought to their lane by one permute
-            {
+            if (!SPLIT || w % nparts == part) {
                 float fx = 0.0f, fy = 0.0f, fz = 0.0f;
                 pair(32, false, fx, fy, fz);
                 ax += (double)fx; ay += (double)fy; az += (double)fz;
@@ -1064,7 +1072,7 @@ struct Chain {
             // (2) the other tiles: 32 offsets each, in two runs of 16; after a
             // run ending at offset kend, lane j's rotated sums are for row
             // (j - 1 + kend) mod 64 of tile t
-            for (int t = 0; t < T; ++t) {
+            for (int t = SPLIT ? part : 0; t < T; t += SPLIT ? nparts : 1) {
                 if (t == w) continue;
                 pa = pa0 + 128 * t + l;
                 pb = pb0 + 128 * t + l;
@@ -1086,9 +1094,10 @@ struct Chain {
         }
         __syncthreads();
         if (own) {
-            y[0] = v[0] + (ax + (double)sm.Acc[0][tid] * inv);
-            y[1] = v[1] + (ay + (double)sm.Acc[1][tid] * inv);
-            y[2] = v[2] + (az + (double)sm.Acc[2][tid] * inv);
+            const double x0 = SPLIT ? 0.0 : v[0], x1 = SPLIT ? 0.0 : v[1], x2 = SPLIT ? 0.0 : v[2];
+            y[0] = x0 + (ax + (double)sm.Acc[0][tid] * inv);
+            y[1] = x1 + (ay + (double)sm.Acc[1][tid] * inv);
+            y[2] = x2 + (az + (double)sm.Acc[2][tid] * inv);
         } else {
             y[0] = y[1] = y[2] = 0.0;
         }
