@@ -65,14 +65,13 @@ struct Smem {
     static constexpr bool PAIRS = MIXED && BLOCK == 64;
     // mixed form, 2..16 waves: the pair walk by 64-bead tiles (apply_tiles)
     static constexpr bool TILES = MIXED && BLOCK >= 128 && BLOCK <= 1024;
-    static constexpr int FB = (MIXED && !PAIRS && !TILES) ? BLOCK : 1;
+    static_assert(!MIXED || PAIRS || TILES, "the mixed form walks bead pairs (64..1024 threads)");
     static constexpr int PB = (PAIRS || TILES) ? 2 * BLOCK : 1;
     static constexpr int AB = TILES ? BLOCK : 1;
     static constexpr int DB = MIXED ? 1 : BLOCK;   // FP64 operand (FP64 form)
     double R[3][BLOCK];         // configuration the mobility is evaluated at
     double P[3][BLOCK];         // force configuration; corrector positions
     double X[3][DB];            // mat-vec operand
-    float Rf[3][FB];            // mixed form, several waves: positions relative to bead 0
     // mixed form, one wave (the pair walk): bead mu's FP32 position relative
     // to bead 0 and the operand's x in Pa[mu], the operand's y, z in Pb[mu],
     // both doubled (entry mu + N repeats mu), so that one 16-byte and one
@@ -614,7 +613,12 @@ __device__ __noinline__ bool chain_barrier(uint32_t* bar, uint32_t target) {
 // and every part adds them in part order. Part 0 writes the results.
 // FT: the type of the LDS factor (NB > 0): double, or float in the mixed
 // form's FP32-factor variant.
-template <int BLOCK, bool MIXED, int NB = 0, bool SPLIT = false, typename FT = double>
+// LWS: the workspace up to its dense region (state slots, Chebyshev
+// coefficients and nodes, inverse-iteration rows, Lanczos basis) lives in
+// LDS (for launches of few chains, whose steps are bound by the latency of
+// these round trips); the dense region stays in HBM.
+template <int BLOCK, bool MIXED, int NB = 0, bool SPLIT = false, typename FT = double,
+          bool LWS = false>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
 #ifndef BDSIM_MIXED_RITZ_F32
@@ -629,7 +633,9 @@ struct Chain {
     const bool own;
     const bool hi;
     const WsLayout wl;
-    gbl_ptr<double> ws;         // this trajectory's workspace
+    using ws_ptr = std::conditional_t<LWS, lds_ptr<double>, gbl_ptr<double>>;
+    ws_ptr ws;                  // this trajectory's workspace (LWS: its part in LDS)
+    gbl_ptr<double> wsg;        // the same in HBM (the dense region is used from here)
     lds_ptr<FT> dlb;            // DenseLds blocks (NB > 0)
     lds_ptr<double> dxs;        // DenseLds operand
     int par = 0;
@@ -639,10 +645,12 @@ struct Chain {
     bool dense_ws = false;      // this step's factor went to the workspace
 
     __device__ __forceinline__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
-                     FT* dense_blk = nullptr, double* dense_xs = nullptr)
+                     FT* dense_blk = nullptr, double* dense_xs = nullptr, double* lds_ws = nullptr)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
-        ws = as_gbl(A.ws + P.ws_off);
+        wsg = as_gbl(A.ws + P.ws_off);
+        if constexpr (LWS) ws = as_lds_mut(lds_ws);
+        else ws = wsg;
         dlb = as_lds_mut(dense_blk);
         dxs = as_lds_mut(dense_xs);
     }
@@ -653,13 +661,36 @@ struct Chain {
     template <typename T, typename U>
     __device__ __forceinline__ void put(T& dst, U v) { if (tid == 0) dst = (T)v; }
 
-    // per-bead state slots in HBM (own bead only; no synchronisation needed)
+    // per-bead state slots (own bead only; no synchronisation needed)
     __device__ __forceinline__ void save(int slot, const double v[3]) {
         if (own)
             for (int d = 0; d < 3; ++d) ws[wl.S + (int64_t)(slot + d) * N + tid] = v[d];
     }
     __device__ __forceinline__ void load(int slot, double v[3]) {
         for (int d = 0; d < 3; ++d) v[d] = own ? ws[wl.S + (int64_t)(slot + d) * N + tid] : 0.0;
+    }
+
+    // LWS: the warm Lanczos seed is the only workspace content a step takes
+    // from the one before; a resumed segment brings it in from HBM and every
+    // launch leaves it there.
+    __device__ __forceinline__ void ws_to_lds() {
+        if constexpr (LWS) {
+            if (own)
+                for (int d = 0; d < 3; ++d) {
+                    const int64_t o = wl.S + (int64_t)(SLOT_SEED + d) * N + tid;
+                    ws[o] = wsg[o];
+                }
+            __syncthreads();
+        }
+    }
+    __device__ __forceinline__ void ws_to_hbm() {
+        if constexpr (LWS) {
+            if (own)
+                for (int d = 0; d < 3; ++d) {
+                    const int64_t o = wl.S + (int64_t)(SLOT_SEED + d) * N + tid;
+                    wsg[o] = ws[o];
+                }
+        }
     }
 
     // Stage the mobility configuration: sm.R from the caller's R, and for the
@@ -691,10 +722,6 @@ struct Chain {
                 sm.Pa[tid + N].x = rx; sm.Pa[tid + N].y = ry; sm.Pa[tid + N].z = rz;
             }
             __syncthreads();
-        } else if constexpr (MIXED) {
-            if (own)
-                for (int d = 0; d < 3; ++d) sm.Rf[d][tid] = (float)(sm.R[d][tid] - sm.R[d][0]);
-            __syncthreads();
         }
     }
 
@@ -703,7 +730,9 @@ struct Chain {
     __device__ void set_part(int p, int G) {
         part = p;
         nparts = G;
-        ws = as_gbl(A.ws + P.ws_off + (int64_t)p * wl.total);
+        static_assert(!LWS, "split chains keep their workspace in HBM");
+        wsg = as_gbl(A.ws + P.ws_off + (int64_t)p * wl.total);
+        ws = wsg;
     }
 
     // SPLIT: the parts' partial rows (own lanes) go to this launch slot's
@@ -785,9 +814,6 @@ struct Chain {
                 sm.Pb[tid] = vyz;
                 sm.Pb[tid + N] = vyz;
             }
-        } else if constexpr (MIXED) {
-            if (own)
-                for (int d = 0; d < 3; ++d) sm.W[d][tid] = (float)v[d];
         } else {
             for (int d = 0; d < 3; ++d) sm.X[d][tid] = v[d];
         }
@@ -807,82 +833,38 @@ struct Chain {
             }
             return;
         }
+        // the FP64 form: each lane over its partners (SPLIT: this part's share)
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (own) {
-            if constexpr (!MIXED) {
-                RpyW<double> W;
-                W.init(P.rpy_d);
-                const lds_cptr<double> r0 = as_lds(sm.R[0]), r1 = as_lds(sm.R[1]), r2 = as_lds(sm.R[2]);
-                const lds_cptr<double> v0 = as_lds(sm.X[0]), v1 = as_lds(sm.X[1]), v2 = as_lds(sm.X[2]);
-                const double rx = r0[tid], ry = r1[tid], rz = r2[tid];
-                const int nlo = SPLIT ? part * N / nparts : 0;
-                const int nhi = SPLIT ? (part + 1) * N / nparts : N;
+            RpyW<double> W;
+            W.init(P.rpy_d);
+            const lds_cptr<double> r0 = as_lds(sm.R[0]), r1 = as_lds(sm.R[1]), r2 = as_lds(sm.R[2]);
+            const lds_cptr<double> v0 = as_lds(sm.X[0]), v1 = as_lds(sm.X[1]), v2 = as_lds(sm.X[2]);
+            const double rx = r0[tid], ry = r1[tid], rz = r2[tid];
+            const int nlo = SPLIT ? part * N / nparts : 0;
+            const int nhi = SPLIT ? (part + 1) * N / nparts : N;
 #pragma unroll 2
-                for (int nu = nlo; nu < nhi; ++nu) {
-                    const double bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
-                    double om1, om2;
-                    W.weights(bx * bx + by * by + bz * bz, om1, om2);
-                    const double xx = v0[nu], xy = v1[nu], xz = v2[nu];
-                    const double t = om2 * (bx * xx + by * xy + bz * xz);
-                    ax = fma(om1, xx, ax); ax = fma(t, bx, ax);
-                    ay = fma(om1, xy, ay); ay = fma(t, by, ay);
-                    az = fma(om1, xz, az); az = fma(t, bz, az);
-                }
-                if constexpr (!SPLIT) {
-                    ax = fma(W.cdiag, v[0], ax);
-                    ay = fma(W.cdiag, v[1], ay);
-                    az = fma(W.cdiag, v[2], az);
-                }
-            } else {
-                RpyW<float> W;
-                W.init(P.rpy_f);
-                const lds_cptr<float> r0 = as_lds(sm.Rf[0]), r1 = as_lds(sm.Rf[1]), r2 = as_lds(sm.Rf[2]);
-                const lds_cptr<float> v0 = as_lds(sm.W[0]), v1 = as_lds(sm.W[1]), v2 = as_lds(sm.W[2]);
-                const float rx = r0[tid], ry = r1[tid], rz = r2[tid];
-                // FP32 pair terms summed over runs of 16 partners, then
-                // accumulated in FP64
-                auto pair = [&](int nu, float& fx, float& fy, float& fz) {
-                    const float bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
-                    float om1, om2;
-                    W.weights(bx * bx + by * by + bz * bz, om1, om2);
-                    const float xx = v0[nu], xy = v1[nu], xz = v2[nu];
-                    const float t = om2 * (bx * xx + by * xy + bz * xz);
-                    fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
-                    fy = fmaf(om1, xy, fy); fy = fmaf(t, by, fy);
-                    fz = fmaf(om1, xz, fz); fz = fmaf(t, bz, fz);
-                };
-                int nu = SPLIT ? part * N / nparts : 0;
-                const int nhi = SPLIT ? (part + 1) * N / nparts : N;
-                for (; nu + 16 <= nhi; nu += 16) {
-                    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) pair(nu + k, fx, fy, fz);
-                    ax += (double)fx;
-                    ay += (double)fy;
-                    az += (double)fz;
-                }
-                if (nu < nhi) {
-                    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-                    for (; nu < nhi; ++nu) pair(nu, fx, fy, fz);
-                    ax += (double)fx;
-                    ay += (double)fy;
-                    az += (double)fz;
-                }
-                if constexpr (!SPLIT) {
-                    const double cd = (double)W.cdiag;
-                    ax = fma(cd, v[0], ax);
-                    ay = fma(cd, v[1], ay);
-                    az = fma(cd, v[2], az);
-                }
+            for (int nu = nlo; nu < nhi; ++nu) {
+                const double bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
+                double om1, om2;
+                W.weights(bx * bx + by * by + bz * bz, om1, om2);
+                const double xx = v0[nu], xy = v1[nu], xz = v2[nu];
+                const double t = om2 * (bx * xx + by * xy + bz * xz);
+                ax = fma(om1, xx, ax); ax = fma(t, bx, ax);
+                ay = fma(om1, xy, ay); ay = fma(t, by, ay);
+                az = fma(om1, xz, az); az = fma(t, bz, az);
+            }
+            if constexpr (!SPLIT) {
+                ax = fma(W.cdiag, v[0], ax);
+                ay = fma(W.cdiag, v[1], ay);
+                az = fma(W.cdiag, v[2], az);
             }
         }
         if constexpr (SPLIT) {
             // every thread of the workgroup takes part in the exchange
             exchange(ax, ay, az);
             if (own) {
-                double cd;
-                if constexpr (MIXED) cd = (double)RpyW<float>::cdiag_of(P.rpy_f);
-                else cd = RpyW<double>::cdiag_of(P.rpy_d);
+                const double cd = RpyW<double>::cdiag_of(P.rpy_d);
                 ax = fma(cd, v[0], ax);
                 ay = fma(cd, v[1], ay);
                 az = fma(cd, v[2], az);
@@ -1318,12 +1300,12 @@ struct Chain {
                 if (dense_factor_lds<BLOCK, MIXED, NB, FT>(sm, dlb, N, P.hstar)) return true;
                 if constexpr (sizeof(FT) == 8) return false;
                 dense_ws = true;
-                const bool ok = dense_factor_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, P.hstar);
+                const bool ok = dense_factor_ool<BLOCK, MIXED>(sm, wsg + wl.dense, N, P.hstar);
                 if (ok) { put(sm.cnt.fail_index, 0); put(sm.cnt.fail_value, 0.0); }
                 return ok;
             }
         }
-        return dense_factor_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, P.hstar);
+        return dense_factor_ool<BLOCK, MIXED>(sm, wsg + wl.dense, N, P.hstar);
     }
 
     // y = L x with the factor where dense_factor left it, own rows.
@@ -1331,9 +1313,9 @@ struct Chain {
         D3 r;
         if constexpr (NB > 0) {
             if (n3 <= 16 * NB && !dense_ws) r = lower_matvec_lds<BLOCK, MIXED, NB, FT>(dlb, dxs, N, x[0], x[1], x[2]);
-            else r = lower_matvec_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, x[0], x[1], x[2]);
+            else r = lower_matvec_ool<BLOCK, MIXED>(sm, wsg + wl.dense, N, x[0], x[1], x[2]);
         } else {
-            r = lower_matvec_ool<BLOCK, MIXED>(sm, ws + wl.dense, N, x[0], x[1], x[2]);
+            r = lower_matvec_ool<BLOCK, MIXED>(sm, wsg + wl.dense, N, x[0], x[1], x[2]);
         }
         y[0] = r.v[0]; y[1] = r.v[1]; y[2] = r.v[2];
     }
@@ -1349,7 +1331,7 @@ struct Chain {
     // vmin + vmax (the next seed is built from their sum only).
     __device__ int lanczos(int m, bool warm, double& rmin, double& rmax, double vsum[3]) {
         if (m > n3) m = n3;
-        const gbl_ptr<double> Vb = ws + wl.V;
+        const ws_ptr Vb = ws + wl.V;
         double v[3], vprev[3] = {0.0, 0.0, 0.0}, w[3];
         if (warm) load(SLOT_SEED, v); else default_seed(v);
         double nv = sqrt(bsum(dot3(v, v)));
@@ -1450,8 +1432,8 @@ struct Chain {
     // Chebyshev coefficients of sqrt on the interval (hydrodynamics.cpp:267-279)
     // into the workspace: one coefficient per thread.
     __device__ void cheb_coefficients(int L, double da, double db) {
-        const gbl_ptr<double> coef = ws + wl.coef;
-        const gbl_ptr<double> aux = ws + wl.aux;
+        const ws_ptr coef = ws + wl.coef;
+        const ws_ptr aux = ws + wl.aux;
         __syncthreads();
         if constexpr (MIXED) {
             // FP32 nodes and sums: cos(pi m / (2(L+1))) = cos(2 pi t) with
@@ -1533,7 +1515,7 @@ struct Chain {
         if (L < 2) L = 2;
         const double da = 2.0 / (lmax - lmin);
         const double db = -(lmax + lmin) / (lmax - lmin);
-        const gbl_ptr<double> coef = ws + wl.coef;
+        const ws_ptr coef = ws + wl.coef;
         double fd = 0.0;
         for (;;) {
             cheb_coefficients(L, da, db);
@@ -1845,6 +1827,28 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)
     c.run(ti);
 }
 
+// 64-thread chains of a small launch with the hot part of the workspace in
+// dynamic LDS (at least 8 ws_layout(N, true).dense bytes, N the largest
+// chain of the launch; chains of the Cholesky method with the factor in LDS
+// are not launched here).
+template <bool MIXED>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) k_integrate_lws(KArgs a) {
+    __shared__ Smem<64, MIXED> sm;
+    extern __shared__ double lws[];
+    const int ti = a.list[blockIdx.x];
+    const TrajParams* __restrict__ P = a.tp + ti;
+    if (a.seg && a.seg[ti].done) {
+        if (threadIdx.x == 0) a.done_out[blockIdx.x] = 1;
+        return;
+    }
+    init_counters(sm);
+    if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
+    Chain<64, MIXED, 0, false, double, true> c(sm, *P, a, nullptr, nullptr, lws);
+    if (a.seg && a.seg[ti].seed_valid) c.ws_to_lds();
+    c.run(ti);
+    c.ws_to_hbm();
+}
+
 // Chains of the Cholesky method with 3N <= 16 NB: the factor in LDS, of
 // type FT.
 template <int BLOCK, bool MIXED, int NB, typename FT = double>
@@ -1964,7 +1968,7 @@ __global__ void __launch_bounds__(BLOCK) k_brownian(KArgs a, const double* x_in,
     const TrajParams* __restrict__ P = a.tp;
     init_counters(sm);
     Chain<BLOCK, MIXED> c(sm, *P, a);
-    c.ws = as_gbl(a.ws + (int64_t)ti * c.wl.total);
+    c.ws = c.wsg = as_gbl(a.ws + (int64_t)ti * c.wl.total);
     const int N = P->N, tid = threadIdx.x;
     const int64_t off = (int64_t)ti * 3 * N;
     double R[3] = {0.0, 0.0, 0.0}, x[3] = {0.0, 0.0, 0.0};

@@ -309,12 +309,32 @@ int dense_nb_for(int N, double hstar, int hi_method) {
     return 0;
 }
 
+// Dynamic LDS for the hot workspace of k_integrate_lws when a launch of
+// 64-thread chains (largest N = nmax) is small enough for it: every chain
+// resident at once at one wave per SIMD with its workspace in LDS, and the
+// launch the only one running (waves = its own). 0: keep the workspace in HBM.
+constexpr size_t LWS_MAX_BYTES = 56 * 1024;
+constexpr size_t LDS_PER_CU = 160 * 1024;
+template <bool MIXED>
+size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
+    static const bool enabled = [] {
+        const char* e = std::getenv("BDSIM_LDS_WORKSPACE");
+        return !(e && e[0] == '0');
+    }();
+    if (!enabled || nmax <= 0 || waves != (long long)n_wg) return 0;
+    const size_t dyn = sizeof(double) * (size_t)ws_layout(nmax, true).dense;
+    if (dyn > LWS_MAX_BYTES) return 0;
+    const size_t per_cu = std::min<size_t>(4, LDS_PER_CU / (sizeof(Smem<64, MIXED>) + dyn));
+    return (size_t)n_wg <= (size_t)g.n_cu * per_cu ? dyn : 0;
+}
+
 // waves: the waves of everything launched together with this grid (buckets
 // side by side), or -1 for this grid alone.
 // f32_factor: the mixed form's variant with the LDS Cholesky factor in FP32.
+// nmax: the largest chain of the grid (0: unknown).
 template <bool MIXED>
 void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, int nb = 0,
-                      long long waves = -1, bool f32_factor = false) {
+                      long long waves = -1, bool f32_factor = false, int nmax = 0) {
     if (waves < 0) waves = (long long)grid.x * (block / 64);
     if (nb == 4 || nb == 6) {
         launch_integrate_small<MIXED>(1, nb, f32_factor, grid, stream, a);
@@ -323,7 +343,8 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     const int ti = tune_index(block);
     const int tuned = ti >= 0 ? g_tune[ti].load() : 0;
     if (block == 64) {
-        launch_integrate_small<MIXED>(tuned ? tuned : wpe_choice(waves), 0, false, grid, stream, a);
+        const size_t lws = tuned ? 0 : lws_bytes_for<MIXED>(nmax, grid.x, waves);
+        launch_integrate_small<MIXED>(tuned ? tuned : wpe_choice(waves), 0, false, grid, stream, a, lws);
         return;
     }
     // Multi-wave chains: the register budget of 4 waves per SIMD when the
@@ -385,9 +406,9 @@ int launch_split(int count, int G, hipStream_t stream, KArgs a, uint32_t* bar, d
 
 template <bool MIXED>
 int launch_bucket(int block, int count, hipStream_t stream, const KArgs& a, int nb, long long waves,
-                  bool f32_factor) {
+                  bool f32_factor, int nmax) {
     if (count == 0) return 0;
-    launch_integrate<MIXED>(block, dim3(count), stream, a, nb, waves, f32_factor);
+    launch_integrate<MIXED>(block, dim3(count), stream, a, nb, waves, f32_factor, nmax);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -426,6 +447,7 @@ struct HostBatch {
     std::vector<ExtEntry> ext;
     std::vector<int32_t> order;
     int counts[N_BUCKETS] = {};
+    int nmax[N_BUCKETS] = {};     // largest chain of each bucket
     int split = 1;
     int64_t roff = 0;
     size_t nlog = 1;
@@ -480,7 +502,11 @@ int plan_batch(int n_traj, const int32_t* nbeads, const double* R, const bdsim_p
     for (int b = 0; b < N_BUCKETS; ++b) {
         const size_t first = hb.order.size();
         for (int i = 0; i < n_traj; ++i)
-            if (bucket[i] == b) { hb.order.push_back(i); ++hb.counts[b]; }
+            if (bucket[i] == b) {
+                hb.order.push_back(i);
+                ++hb.counts[b];
+                hb.nmax[b] = std::max(hb.nmax[b], (int)nbeads[i]);
+            }
         std::stable_sort(hb.order.begin() + first, hb.order.end(),
                          [&](int32_t x, int32_t y) { return nbeads[x] > nbeads[y]; });
     }
@@ -567,9 +593,9 @@ int run_batch(const HostBatch& hb, const BatchBufs& d, double* R, bdsim_rng_stat
         } else {
             rc = is_mixed(precision)
                      ? launch_bucket<true>(BUCKET_BLOCK[b], hb.counts[b], g.bstream[b], a, BUCKET_DNB[b], waves,
-                                           precision == BDSIM_PRECISION_MIXED_F32_FACTOR)
+                                           precision == BDSIM_PRECISION_MIXED_F32_FACTOR, hb.nmax[b])
                      : launch_bucket<false>(BUCKET_BLOCK[b], hb.counts[b], g.bstream[b], a, BUCKET_DNB[b], waves,
-                                            false);
+                                            false, hb.nmax[b]);
         }
         if (rc) {
             (void)hipDeviceSynchronize();     // buckets already queued must end before reuse
@@ -862,8 +888,8 @@ int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
                    : launch_split<false>(b->n_traj, b->split, b->stream, a, b->bar, b->xbuf);
     if (is_mixed(b->precision))
         launch_integrate<true>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb, -1,
-                               b->precision == BDSIM_PRECISION_MIXED_F32_FACTOR);
-    else launch_integrate<false>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb);
+                               b->precision == BDSIM_PRECISION_MIXED_F32_FACTOR, b->N);
+    else launch_integrate<false>(b->block, dim3(b->n_traj), b->stream, a, b->dense_nb, -1, false, b->N);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -922,8 +948,8 @@ int bdsim_batch_run_steps(bdsim_batch* b, const bdsim_sim* sim, int max_steps, i
     a.max_steps = max_steps;
     if (is_mixed(b->precision))
         launch_integrate<true>(b->block, dim3(n), b->stream, a, b->dense_nb, -1,
-                               b->precision == BDSIM_PRECISION_MIXED_F32_FACTOR);
-    else launch_integrate<false>(b->block, dim3(n), b->stream, a, b->dense_nb);
+                               b->precision == BDSIM_PRECISION_MIXED_F32_FACTOR, b->N);
+    else launch_integrate<false>(b->block, dim3(n), b->stream, a, b->dense_nb, -1, false, b->N);
     HIP_TRY(hipGetLastError());
     return 0;
 }

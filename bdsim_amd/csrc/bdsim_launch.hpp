@@ -11,9 +11,11 @@ namespace bdgpu {
 // register allocation is held to: 1, 2 or 4), or with nb = 4 / 6 the Cholesky
 // kernel with its factor in LDS, k_integrate_dense<64, MIXED, nb, FT>, FT
 // float for the mixed form's FP32-factor variant (f32_factor), else double.
+// lws_bytes > 0: k_integrate_lws<MIXED> with that much dynamic LDS for the
+// hot part of the workspace (few chains; nb = 0).
 template <bool MIXED>
 void launch_integrate_small(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
-                            const KArgs& a);
+                            const KArgs& a, size_t lws_bytes = 0);
 
 // Multi-wave chains: k_integrate<block, MIXED, wpe>, block 128..1024, wpe 1 or
 // 4 (1024: 1).

@@ -7,7 +7,11 @@ namespace bdgpu {
 
 template <>
 void launch_integrate_small<false>(int wpe, int nb, bool, dim3 grid, hipStream_t stream,
-                                    const KArgs& a) {
+                                    const KArgs& a, size_t lws_bytes) {
+    if (lws_bytes > 0 && nb == 0) {
+        hipLaunchKernelGGL((k_integrate_lws<false>), grid, dim3(64), lws_bytes, stream, a);
+        return;
+    }
     if (nb == 4) hipLaunchKernelGGL((k_integrate_dense<64, false, 4>), grid, dim3(64), 0, stream, a);
     else if (nb == 6) hipLaunchKernelGGL((k_integrate_dense<64, false, 6>), grid, dim3(64), 0, stream, a);
     else if (wpe == 4) hipLaunchKernelGGL((k_integrate<64, false, 4>), grid, dim3(64), 0, stream, a);

@@ -7,7 +7,11 @@ namespace bdgpu {
 
 template <>
 void launch_integrate_small<true>(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
-                                   const KArgs& a) {
+                                   const KArgs& a, size_t lws_bytes) {
+    if (lws_bytes > 0 && nb == 0) {
+        hipLaunchKernelGGL((k_integrate_lws<true>), grid, dim3(64), lws_bytes, stream, a);
+        return;
+    }
     if (nb == 4 && f32_factor) hipLaunchKernelGGL((k_integrate_dense<64, true, 4, float>), grid, dim3(64), 0, stream, a);
     else if (nb == 6 && f32_factor) hipLaunchKernelGGL((k_integrate_dense<64, true, 6, float>), grid, dim3(64), 0, stream, a);
     else if (nb == 4) hipLaunchKernelGGL((k_integrate_dense<64, true, 4>), grid, dim3(64), 0, stream, a);

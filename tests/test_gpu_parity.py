@@ -460,6 +460,33 @@ def test_resident_resume_needs_its_segment(gpu):
         gpu.bdsim_batch_destroy(b)
 
 
+def test_small_launch_with_lds_workspace(gpu, orc):
+    """A launch of few short chains keeps the hot workspace (state slots,
+    Chebyshev coefficients, Lanczos basis) in LDS: the 16 chains of N=10 of the
+    CPU-sized case, 100 steps, FP64 within 1e-10 of the oracle with its
+    per-step decisions and generator states, mixed within 1e-5."""
+    from bdsim_amd.workloads import c1_workload
+    R0, phys, _, _, _ = c1_workload()
+    n, N = R0.shape[0], R0.shape[1]
+    pk = A.PackedPhys(phys, N)
+    sim = A.CSim(0.0, 0.99, 0.01, 1e-3, 1)
+    items = [(N, R0[i], pk.c, sim, H.rng_of(i)) for i in range(n)]
+    out, rngs, status, cnts, logs = H.gpu_batch(gpu, items, log_stride=100)
+    mixed, _, status_m, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    assert np.all(status == 0) and np.all(status_m == 0)
+    for i in range(n):
+        st = H.rng_of(i)
+        want, rc, cnto, lo = H.oracle_integrate(orc, N, R0[i], pk.c, sim, st, log_cap=100)
+        assert rc == 0 and cnts[i].steps == cnto.steps == 100
+        assert rel(out[i], want) <= 1e-10, (i, rel(out[i], want))
+        assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st.ix, st.iy, st.seed)
+        for k in range(100):
+            a, b = logs[i * 100 + k], lo[k]
+            assert (a.lanczos_cold, a.lanczos_steps, a.cheb_L, a.cheb_attempts, a.corrector_iters) == \
+                (b.lanczos_cold, b.lanczos_steps, b.cheb_L, b.cheb_attempts, b.corrector_iters), (i, k)
+        assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))
+
+
 def test_mixed_precision_within_1e5(gpu, orc):
     """Headline physics (N=50, FENE b=100, h*=0.3 Chebyshev, SDK, shear):
     the mixed form stays within 1e-5 relative of the FP64 oracle over 100 steps."""
