@@ -313,7 +313,9 @@ int dense_nb_for(int N, double hstar, int hi_method) {
 // 64-thread chains (largest N = nmax) is small enough for it: every chain
 // resident at once at one wave per SIMD with its workspace in LDS, and the
 // launch the only one running (waves = its own). 0: keep the workspace in HBM.
-constexpr size_t LWS_MAX_BYTES = 56 * 1024;
+// A workgroup's static and dynamic LDS together stay within 64 KB (the
+// dynamic size a launch may ask for without a kernel attribute).
+constexpr size_t LWS_MAX_TOTAL = 64 * 1024;
 constexpr size_t LDS_PER_CU = 160 * 1024;
 template <bool MIXED>
 size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
@@ -323,7 +325,7 @@ size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
     }();
     if (!enabled || nmax <= 0 || waves != (long long)n_wg) return 0;
     const size_t dyn = sizeof(double) * (size_t)ws_layout(nmax, true).dense;
-    if (dyn > LWS_MAX_BYTES) return 0;
+    if (sizeof(Smem<64, MIXED>) + dyn > LWS_MAX_TOTAL) return 0;
     const size_t per_cu = std::min<size_t>(4, LDS_PER_CU / (sizeof(Smem<64, MIXED>) + dyn));
     return (size_t)n_wg <= (size_t)g.n_cu * per_cu ? dyn : 0;
 }

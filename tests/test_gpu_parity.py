@@ -487,6 +487,28 @@ def test_small_launch_with_lds_workspace(gpu, orc):
         assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))
 
 
+def test_lds_workspace_at_its_largest(gpu, orc):
+    """The largest chains whose hot workspace still goes to LDS (N=54: 64 KB
+    of LDS per workgroup with the static part), three of them, headline
+    physics, 30 steps: FP64 within 1e-10 of the oracle, mixed within 1e-5."""
+    N, n = 54, 3
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.3, ev=3, dstar=1.0, kappa=C.wi_kappa(10.0))
+    p = H.Packed(d, N)
+    sim = H.csim(C.sim_dict(0.0, 0.029, 1e-3, 1e-3))
+    R0 = [C.random_walk(N, 60 + i) for i in range(n)]
+    items = [(N, R0[i], p.c, sim, H.rng_of(i)) for i in range(n)]
+    out, rngs, status, cnts, _ = H.gpu_batch(gpu, items)
+    mixed, _, status_m, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    assert np.all(status == 0) and np.all(status_m == 0)
+    for i in range(n):
+        st = H.rng_of(i)
+        want, rc, cnto, _ = H.oracle_integrate(orc, N, R0[i], p.c, sim, st)
+        assert rc == 0 and cnts[i].steps == cnto.steps == 30
+        assert rel(out[i], want) <= 1e-10, (i, rel(out[i], want))
+        assert (rngs[i].ix, rngs[i].iy, rngs[i].seed) == (st.ix, st.iy, st.seed)
+        assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))
+
+
 def test_mixed_precision_within_1e5(gpu, orc):
     """Headline physics (N=50, FENE b=100, h*=0.3 Chebyshev, SDK, shear):
     the mixed form stays within 1e-5 relative of the FP64 oracle over 100 steps."""
