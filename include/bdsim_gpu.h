@@ -140,8 +140,12 @@ typedef struct bdsim_sim {
 typedef struct bdsim_rng_state { int32_t ix, iy, seed; } bdsim_rng_state;
 
 /* Per-trajectory work counters of one integrate call. matvecs counts every
- * 3N x 3N mobility product D.x (Lanczos, D.x0, Chebyshev terms, predictor,
- * upsilon, corrector); it is the n_mv of the roofline accounting.           */
+ * 3N x 3N mobility product D.x the device forms (Lanczos, D.x0, Chebyshev
+ * terms, predictor, upsilon, corrector); it is the n_mv of the roofline
+ * accounting. Per step it is two fewer than the reference forms when the
+ * Chebyshev square root runs: D.x0 is formed once for the FD test and the
+ * series, and the upsilon's two products are one product of the summed
+ * forces (D is linear).                                                     */
 typedef struct bdsim_counters {
     int64_t steps;
     int64_t matvecs;
