@@ -695,7 +695,7 @@ struct Chain {
 
     // Stage the mobility configuration: sm.R from the caller's R, and for the
     // mixed form its FP32 copy relative to bead 0.
-    __device__ void stage_R(const double R[3]) {
+    __device__ __forceinline__ void stage_R(const double R[3]) {
         __syncthreads();
         for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
         __syncthreads();
@@ -727,7 +727,7 @@ struct Chain {
 
     // SPLIT: this workgroup is part p of G of its chain; its state copy lives
     // in its own workspace
-    __device__ void set_part(int p, int G) {
+    __device__ __forceinline__ void set_part(int p, int G) {
         part = p;
         nparts = G;
         static_assert(!LWS, "split chains keep their workspace in HBM");
@@ -752,7 +752,7 @@ struct Chain {
                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
 
-    __device__ void exchange(double& ax, double& ay, double& az) {
+    __device__ __forceinline__ void exchange(double& ax, double& ay, double& az) {
         const int slot = blockIdx.x / nparts;
         double* xb = A.xbuf + ((size_t)slot * 2 + (epoch & 1)) * (size_t)nparts * 3 * 1024;
         if (own) {
@@ -1067,9 +1067,15 @@ struct Chain {
                     for (int u = 0; u < 16; ++u) pair(k0 - u, true, gx, gy, gz);
                     ax += (double)gx; ay += (double)gy; az += (double)gz;
                     const int row = 64 * t + ((l + k0 - 16) & 63);
-                    atomicAdd((unsigned long long*)&sm.Acc[0][row], (unsigned long long)(long long)__float2ll_rn(sx * FIX));
-                    atomicAdd((unsigned long long*)&sm.Acc[1][row], (unsigned long long)(long long)__float2ll_rn(sy * FIX));
-                    atomicAdd((unsigned long long*)&sm.Acc[2][row], (unsigned long long)(long long)__float2ll_rn(sz * FIX));
+                    // LDS-qualified, so that these are ds_add_u64 (not flat atomics)
+                    const lds_ptr<unsigned long long> acc =
+                        as_lds_mut(reinterpret_cast<unsigned long long*>(&sm.Acc[0][0])) + row;
+                    __hip_atomic_fetch_add(acc, (unsigned long long)(long long)__float2ll_rn(sx * FIX),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(acc + S::AB, (unsigned long long)(long long)__float2ll_rn(sy * FIX),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(acc + 2 * S::AB, (unsigned long long)(long long)__float2ll_rn(sz * FIX),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     sx = sy = sz = 0.0f;
                 }
             }
@@ -1087,7 +1093,7 @@ struct Chain {
 
     // Spring force on the beads from the bonds held by threads tid < N-1
     // (spring.cpp:183-207). Returns the bead force; Fc gets the own bond force.
-    __device__ void spring_bead_force(const double bond[3], double Fc[3], double F[3]) {
+    __device__ __forceinline__ void spring_bead_force(const double bond[3], double Fc[3], double F[3]) {
         Fc[0] = Fc[1] = Fc[2] = 0.0;
         if (tid < N - 1) connector_force(P, bond, Fc);
         __syncthreads();
@@ -1103,7 +1109,7 @@ struct Chain {
     }
 
     // Bending force (bending.cpp:5-43) at configuration C (LDS, [3][BLOCK]).
-    __device__ void bending_force(const double (*C)[BLOCK], double F[3]) {
+    __device__ __forceinline__ void bending_force(const double (*C)[BLOCK], double F[3]) {
         F[0] = F[1] = F[2] = 0.0;
         if (P.bend_type == 0 || N == 2) return;
         const double Cst = P.bend_C;
@@ -1174,7 +1180,7 @@ struct Chain {
     // q = 0 there, so its term vanishes whatever the (finite) factor. The
     // mixed form works on FP32 positions relative to bead 0, with FP32 pair
     // terms summed over runs of 8 partners and accumulated in FP64.
-    __device__ void ev_force(const double (*C)[BLOCK], double F[3]) {
+    __device__ __forceinline__ void ev_force(const double (*C)[BLOCK], double F[3]) {
         F[0] = F[1] = F[2] = 0.0;
         const int et = P.ev_type;
         if (et != 1 && et != 2 && et != 3) return;
@@ -1281,7 +1287,7 @@ struct Chain {
     }
 
     // non-spring + external force at C and time t (model.hpp:56-64)
-    __device__ void explicit_force(const double (*C)[BLOCK], double t, double Fe[3]) {
+    __device__ __forceinline__ void explicit_force(const double (*C)[BLOCK], double t, double Fe[3]) {
         double Fev[3], Fb[3], Fx[3] = {0.0, 0.0, 0.0};
         ev_force(C, Fev);
         bending_force(C, Fb);
@@ -1329,7 +1335,7 @@ struct Chain {
     // from the seed slot (warm) or the default seed. Returns the Krylov
     // dimension; extremal Ritz values out, Ritz vectors into the seed slot as
     // vmin + vmax (the next seed is built from their sum only).
-    __device__ int lanczos(int m, bool warm, double& rmin, double& rmax, double vsum[3]) {
+    __device__ __forceinline__ int lanczos(int m, bool warm, double& rmin, double& rmax, double vsum[3]) {
         if (m > n3) m = n3;
         const ws_ptr Vb = ws + wl.V;
         double v[3], vprev[3] = {0.0, 0.0, 0.0}, w[3];
@@ -1431,7 +1437,7 @@ struct Chain {
 
     // Chebyshev coefficients of sqrt on the interval (hydrodynamics.cpp:267-279)
     // into the workspace: one coefficient per thread.
-    __device__ void cheb_coefficients(int L, double da, double db) {
+    __device__ __forceinline__ void cheb_coefficients(int L, double da, double db) {
         const ws_ptr coef = ws + wl.coef;
         const ws_ptr aux = ws + wl.aux;
         __syncthreads();
@@ -1472,7 +1478,7 @@ struct Chain {
     // B.x0 by the Chebyshev method with Lanczos bounds and FD control
     // (hydrodynamics.cpp:224-337, mobility.cpp:64-80), x0 in SLOT_X0. Returns
     // false when the fallback factorisation fails.
-    __device__ bool chebyshev_bx0(bool& seed_valid, double bx0[3]) {
+    __device__ __forceinline__ bool chebyshev_bx0(bool& seed_valid, double bx0[3]) {
         StepLog& lg = sm.lg;
         double rmin, rmax, vsum[3];
         bool cold = !seed_valid;
@@ -1562,7 +1568,7 @@ struct Chain {
     // spectral cache (mobility.cpp:58-88 without the sqrt(dt) factor): x0 for
     // free draining, L x0 for Cholesky, the Chebyshev series of sqrt(D) x0 on
     // cold Lanczos bounds. For parity checks of the forms.
-    __device__ int brownian(const double x0[3], double bx0[3]) {
+    __device__ __forceinline__ int brownian(const double x0[3], double bx0[3]) {
         if (!hi) {
             for (int d = 0; d < 3; ++d) bx0[d] = x0[d];
             return ST_OK;
