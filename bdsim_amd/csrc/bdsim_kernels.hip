@@ -909,10 +909,11 @@ int bdsim_batch_run_steps(bdsim_batch* b, const bdsim_sim* sim, int max_steps, i
             (sim->update_center_of_mass != 0) != (o.update_center_of_mass != 0))
             return fail(BDSIM_EINVAL, "resume with the segment's own bdsim_sim");
     }
-    if (!b->seg &&
-        (hipMalloc(&b->seg, sizeof(SegState) * b->n_traj) != hipSuccess ||
-         hipMalloc(&b->done, sizeof(int32_t) * b->n_traj) != hipSuccess))
-        return fail(BDSIM_ENOMEM, "hipMalloc failed for the segment state");
+    if (!b->seg || !b->done) {
+        if (!b->seg && hipMalloc(&b->seg, sizeof(SegState) * b->n_traj) != hipSuccess) b->seg = nullptr;
+        if (!b->done && hipMalloc(&b->done, sizeof(int32_t) * b->n_traj) != hipSuccess) b->done = nullptr;
+        if (!b->seg || !b->done) return fail(BDSIM_ENOMEM, "hipMalloc failed for the segment state");
+    }
     const int n = b->n_traj;
     if (!resume) {
         // counters run on across the launches of one segment: start them at zero
