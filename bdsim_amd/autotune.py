@@ -1,16 +1,22 @@
-"""Per-bucket register budgets of the integrate kernels, picked by timing.
+"""Per-bucket launch shapes of the integrate kernels, picked by timing.
 
 The library launches the chains of a batch in buckets by block size (64, 128,
 256, 512 threads for chains of up to that many beads). Each bucket's kernel
 exists in builds for different register budgets (waves per SIMD: 1, 2 or 4 for
 64-thread chains; 1 or 4 for the others); more registers per wave means fewer
-spills, fewer means more chains resident at once. Which wins depends on the
-mix, so `tune` times the candidates on a representative batch -- one bucket at
-a time, the others on the library's own choice, all buckets running side by
-side as they do in use -- and keeps the fastest. The table applies with
-`apply` (bdsim_tune_set) and round-trips through JSON (`save` / `load`).
-Results do not depend on the budget (the same operations, another register
-allocation), so tuning changes time only.
+spills, fewer means more chains resident at once. Chains of more than 512
+beads, when they alone leave compute units idle, run split over several
+workgroups each (waves per chain: 16 per workgroup times G workgroups); more
+parts means less work per part but more exchanges to wait for. Which wins
+depends on the mix, so `tune` times the candidates on a representative batch
+-- one bucket at a time, the others on the library's own choice, all buckets
+running side by side as they do in use -- and keeps the fastest, and for
+batches of long chains the largest G. The table ({block: waves per SIMD},
+and under key 1024 the workgroups per long chain) applies with `apply`
+(bdsim_tune_set, bdsim_tune_split) and round-trips through JSON (`save` /
+`load`). Register budgets change time only (the same operations, another
+register allocation); G also changes the order in which partner sums are
+added.
 """
 import json
 import time
@@ -20,6 +26,8 @@ import numpy as np
 from . import _abi
 
 CANDIDATES = {64: (1, 2, 4), 128: (1, 4), 256: (1, 4), 512: (1, 4)}
+SPLIT_KEY = 1024                      # table key of the workgroups per long chain
+SPLIT_CANDIDATES = (1, 2, 4, 8, 16)
 
 
 def block_of(N):
@@ -31,19 +39,25 @@ def block_of(N):
 
 
 def apply(table, lib=None):
-    """Set the library's budgets from {block: waves} (0 = automatic)."""
+    """Set the library's budgets from {block: waves} and the workgroups per
+    long chain from table[1024] (0 = automatic)."""
     lib = lib or _abi.load_gpu()
     for block, w in table.items():
-        _abi.check(lib, lib.bdsim_tune_set(int(block), int(w)))
+        if int(block) == SPLIT_KEY:
+            _abi.check(lib, lib.bdsim_tune_split(int(w)))
+        else:
+            _abi.check(lib, lib.bdsim_tune_set(int(block), int(w)))
 
 
 def reset(lib=None):
-    apply({b: 0 for b in CANDIDATES}, lib)
+    apply({**{b: 0 for b in CANDIDATES}, SPLIT_KEY: 0}, lib)
 
 
 def current(lib=None):
     lib = lib or _abi.load_gpu()
-    return {b: int(lib.bdsim_tune_get(b)) for b in CANDIDATES}
+    out = {b: int(lib.bdsim_tune_get(b)) for b in CANDIDATES}
+    out[SPLIT_KEY] = int(lib.bdsim_tune_split_get())
+    return out
 
 
 def save(table, path):
@@ -79,12 +93,27 @@ def _timed(lib, N, R, phys, dt, precision, steps, warmup):
 def tune(N, R, phys, dt, precision=_abi.PRECISION_MIXED, steps=20, warmup=3, repeats=2,
          lib=None, verbose=False):
     """Fastest budget per bucket present in the batch (N: chain lengths, R:
-    their positions, phys: CPhys per chain, dt). Returns {block: waves}; the
-    library is left on that table."""
+    their positions, phys: CPhys per chain, dt) and, for a batch of chains of
+    more than 512 beads only, the fastest number of workgroups per chain.
+    Returns {block: waves} (with 1024: workgroups per chain); the library is
+    left on that table."""
     lib = lib or _abi.load_gpu()
     blocks = sorted({block_of(n) for n in N if block_of(n) in CANDIDATES})
     table = {}
     reset(lib)
+    # one untimed pass: the first integrate call of a form in a process is slow
+    _timed(lib, N, R, phys, dt, precision, steps, warmup)
+    if len(N) > 0 and all(block_of(n) == SPLIT_KEY for n in N):
+        best = None
+        for G in SPLIT_CANDIDATES:
+            apply({SPLIT_KEY: G}, lib)
+            t = min(_timed(lib, N, R, phys, dt, precision, steps, warmup) for _ in range(repeats))
+            if verbose:
+                print(f"autotune: {G} workgroups per chain: {t * 1e3 / steps:.3f} ms/step")
+            if best is None or t < best[1]:
+                best = (G, t)
+        table[SPLIT_KEY] = best[0]
+        apply({SPLIT_KEY: best[0]}, lib)
     for b in blocks:
         best = None
         for w in CANDIDATES[b]:

@@ -357,16 +357,23 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     launch_integrate_large<MIXED>(block, four && block < 1024 ? 4 : 1, grid, stream, a);
 }
 
+// Largest number of workgroups per split chain set by the caller
+// (bdsim_tune_split): 0 = the library's choice (8; every part holds its own
+// workspace copy, 9 N^2 doubles of it the dense region).
+std::atomic<int> g_split_max{0};
+
 // Workgroups per chain for a batch whose only chains are `count` chains of
-// N > 512 (1 = no split): as many as fill the compute units, at most 8.
-// BDSIM_SPLIT=0 in the environment turns the split off.
+// N > 512 (1 = no split): as many as fill the compute units, at most 8 (or
+// the tuned maximum, up to 16). BDSIM_SPLIT=0 in the environment turns the
+// split off.
 int split_factor(int count, int total) {
     static const bool enabled = [] {
         const char* e = std::getenv("BDSIM_SPLIT");
         return !(e && e[0] == '0');
     }();
     if (!enabled || !g.coop || count == 0 || count != total) return 1;
-    const int G = std::min(8, g.n_cu / count);
+    const int cap = g_split_max.load();
+    const int G = std::min(cap > 0 ? cap : 8, g.n_cu / count);
     return G >= 2 ? G : 1;
 }
 
@@ -698,6 +705,15 @@ int bdsim_tune_get(int block) {
     const int i = tune_index(block);
     return i < 0 ? -1 : g_tune[i].load();
 }
+
+int bdsim_tune_split(int max_workgroups) {
+    if (max_workgroups < 0 || max_workgroups > 16)
+        return fail(BDSIM_EINVAL, "tune: workgroups per chain must be 0 (automatic) or 1..16");
+    g_split_max.store(max_workgroups);
+    return 0;
+}
+
+int bdsim_tune_split_get(void) { return g_split_max.load(); }
 
 int bdsim_integrate_batch(int n_traj, const int32_t* nbeads, double* R, const bdsim_phys* phys,
                           const bdsim_sim* sim, bdsim_rng_state* rng, int precision,

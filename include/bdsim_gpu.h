@@ -198,6 +198,15 @@ int bdsim_max_beads(void);
  * in the reference.                                                          */
 int bdsim_tune_set(int block, int waves_per_simd);
 int bdsim_tune_get(int block);           /* current setting, -1 for a bad block */
+/* Workgroups per chain for batches of chains of N > 512 that leave compute
+ * units idle (the chain split over co-resident workgroups): at most
+ * max_workgroups (1 = no split, 2..16), 0 restores the library's choice (as
+ * many as fill the compute units, at most 8). Every part holds its own
+ * workspace copy (about 8 (9 N^2 + 120 N) bytes). Results do not depend on it
+ * beyond rounding of the partial-row sums (each part sums 1/G of the
+ * partners). Set by bdsim_amd.autotune; no counterpart in the reference.     */
+int bdsim_tune_split(int max_workgroups);
+int bdsim_tune_split_get(void);
 
 /* ---- the step: batched time_integrate_chain ---------------------------- */
 

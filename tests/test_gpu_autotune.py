@@ -50,3 +50,35 @@ def test_tune_rejects_bad_settings(gpu):
     assert gpu.bdsim_tune_set(128, 2) != 0
     assert gpu.bdsim_tune_set(64, 3) != 0
     assert gpu.bdsim_tune_get(96) == -1
+
+
+def test_tune_workgroups_per_long_chain(gpu, orc):
+    """Batches of chains of N > 512 only: the tuner times 1, 2, 4, 8 and 16
+    workgroups per chain and keeps one; every choice integrates to the
+    oracle's positions within 1e-10 (the parts only reorder the partner
+    sums)."""
+    import helpers as H
+    from golden import cases as C
+    N_, n = 600, 3
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=1, zstar=0.3, dstar=0.5)
+    p = H.Packed(d, N_)
+    R = [W.self_avoiding_walk(N_, np.random.default_rng(90 + i)) for i in range(n)]
+    N = np.full(n, N_, dtype=np.int32)
+    try:
+        table = autotune.tune(N, R, [p.c] * n, 1e-3, precision=A.PRECISION_FP64, steps=2, warmup=1,
+                              repeats=1, lib=gpu)
+        assert table[autotune.SPLIT_KEY] in autotune.SPLIT_CANDIDATES
+        assert autotune.current(gpu)[autotune.SPLIT_KEY] == table[autotune.SPLIT_KEY]
+        sim = A.CSim(0.0, 0.004, 1e-3, 1e-3, 1)
+        want = [H.oracle_integrate(orc, N_, R[i], p.c, sim, H.rng_of(i))[0] for i in range(n)]
+        for G in autotune.SPLIT_CANDIDATES:
+            autotune.apply({autotune.SPLIT_KEY: G}, gpu)
+            out, _, status, _, _ = H.gpu_batch(gpu, [(N_, R[i], p.c, sim, H.rng_of(i)) for i in range(n)])
+            assert np.all(status == 0)
+            for i in range(n):
+                err = np.max(np.abs(out[i] - want[i])) / np.max(np.abs(want[i]))
+                assert err <= 1e-10, (G, i, err)
+    finally:
+        autotune.reset(gpu)
+    assert gpu.bdsim_tune_split_get() == 0
+    assert gpu.bdsim_tune_split(17) != 0 and gpu.bdsim_tune_split(-1) != 0
