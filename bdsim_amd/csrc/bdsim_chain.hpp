@@ -586,6 +586,223 @@ __device__ __noinline__ D3 lower_matvec_lds(lds_ptr<T> Lb, lds_ptr<double> xs, i
     return y;
 }
 
+// ---- the dense factor held in registers (Cholesky chains, FP64 factor) ------
+// The padded tensor (16 NB rows; unit rows and columns beyond 3N) is held as
+// its upper blocks J <= I of 16 x 16 in the accumulator layout of the FP64
+// 16x16x4 product: lane l = 16 g + c holds entries (16 J + g + 4 q, 16 I + c),
+// q = 0..3. Read in that layout, value kk of a block is chunk kk of the
+// product's B operand, and of its A operand for the block's transpose, so the
+// update A_KI -= U_JK^T U_JI is four products with no data movement. Block
+// rows are formed left-looking, and each is factored unblocked in the upper
+// form (U = L^T; row j of U: the pivot, the scaled row across all NB - K
+// blocks at once, the rank-1 update of the rows below; linalg.cpp:91-107 by
+// rows of U), one entry per lane and value, the row passed through a small
+// LDS buffer. L x = U^T x accumulates block row by block row, so a block dies
+// after its last use: at most 12 of the 21 blocks are live at NB = 6 (96
+// VGPRs), which with the rest of the step fits the register budget of two
+// waves per SIMD, and the chain needs 13 KB of LDS instead of 53 KB.
+template <int NB>
+struct RegLds {
+    double row[NB][16];         // row j of each block of block row K, unscaled
+    double xs[16 * NB];         // the operand, zero beyond 3N
+    double yp[4][16 * NB];      // L x: partial sums of the four lane row groups
+};
+
+// The tensor's entries (as dense_factor_ool forms them, one reciprocal square
+// root per entry) from the FP64 positions in LDS, and the lane's place.
+struct RegBuild {
+    lds_cptr<double> R0, R1, R2;
+    int n, c, g;
+    double hstar, ihstar;
+    // block (J, I) in the accumulator layout
+    __device__ __forceinline__ mfma_d4 operator()(int J, int I) const {
+        const double RPI = 1.7724538509055160273;
+        const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
+                     C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75;
+        // the lane's place as an opaque value: otherwise the index arithmetic
+        // of all 21 blocks is hoisted out of the step loop and held (spilled)
+        // across the whole step
+        int cl = c, gl = g;
+        asm volatile("" : "+v"(cl), "+v"(gl));
+        const int j = 16 * I + cl;
+        const int jb = j / 3;
+        const int qc = j - 3 * jb;
+        const int nu = j < n ? jb : 0;
+        const double rn0 = R0[nu], rn1 = R1[nu], rn2 = R2[nu];
+        mfma_d4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * J + gl + 4 * q;
+            const int ib = i / 3;
+            const int p = i - 3 * ib;
+            const int mu = i < n ? ib : 0;
+            const double b0 = R0[mu] - rn0, b1 = R1[mu] - rn1, b2 = R2[mu] - rn2;
+            double sq = b0 * b0 + b1 * b1 + b2 * b2;
+            if (sq < 1.0e-12) sq = 1.0e-12;
+            const double ir = rsqrt(sq);
+            const double ir2 = ir * ir;
+            const double rs = sq * ir;
+            const double hsbyrs = hstar * ir, rsbyhs = rs * ihstar;
+            const double hsbyrs2 = hsbyrs * hsbyrs;
+            double om1, om2;
+            if (rsbyhs >= TRPI) {
+                om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
+                om2 = RPI3by4 * (hsbyrs * ir2) * (1.0 - TPI * hsbyrs2);
+            } else {
+                om1 = 1.0 - C3 * rsbyhs;
+                om2 = C7 * rsbyhs * ir2;
+            }
+            const double bp = p == 0 ? b0 : (p == 1 ? b1 : b2);
+            const double bq = qc == 0 ? b0 : (qc == 1 ? b1 : b2);
+            double e = om2 * bq * bp + (p == qc ? om1 : 0.0);
+            // one bead's own 3 x 3 block and the padding: the identity
+            if (i >= n || j >= n || ib == jb) e = i == j ? 1.0 : 0.0;
+            // formed here: sunk to its first use, the entry would keep its
+            // separation and weights live across the whole block row
+            asm volatile("" : "+v"(e));
+            v[q] = e;
+        }
+        return v;
+    }
+};
+
+// Block row K of the register factor, left-looking: the row's blocks built
+// and updated by the rows above (A_KI - sum_J U_JK^T U_JI, four products per
+// term, in increasing J), its rows of U, and its part of L x. Recursion over
+// K keeps every block index a compile-time constant, so the blocks stay in
+// registers; U_JK dies once row K is formed, so (K + 1)(NB - K) blocks are
+// live at most (12 at NB = 6). false on a non-positive pivot.
+template <int K, int NB>
+__device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], double (&ypart)[NB], const RegBuild& build,
+                                               lds_ptr<double> rowb, lds_cptr<double> xs, Counters& cnt) {
+    if constexpr (K == NB) {
+        return true;
+    } else {
+        const int n = build.n, c = build.c, g = build.g;
+#pragma unroll
+        for (int I = K; I < NB; ++I) M[K][I] = build(K, I);
+#pragma unroll
+        for (int J = 0; J < K; ++J) {
+            mfma_d4 na;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) na[qq] = -M[J][K][qq];
+#pragma unroll
+            for (int I = K; I < NB; ++I) {
+                mfma_d4 acc = M[K][I];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) acc = mfma16(na[kk], M[J][I][kk], acc);
+                M[K][I] = acc;
+            }
+        }
+        // rows jr = 4 q + gj of block row K, in order (rows of padding stay
+        // unit rows); row jr is held by the lanes of row group gj, value q
+        const int rows = min(16, n - 16 * K);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ng = min(4, rows - 4 * q);
+            for (int gj = 0; gj < ng; ++gj) {
+                const int jr = 4 * q + gj;
+                if (g == gj) {
+#pragma unroll
+                    for (int I = K; I < NB; ++I) rowb[16 * I + c] = M[K][I][q];
+                }
+                __syncthreads();
+                double piv = rowb[16 * K + jr];
+                piv = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(piv)),
+                                       __builtin_amdgcn_readfirstlane(__double2loint(piv)));
+                if (!(piv > 0.0)) {
+                    if (threadIdx.x == 0) {
+                        cnt.fail_index = 16 * K + jr;
+                        cnt.fail_value = piv;
+                    }
+                    __syncthreads();
+                    return false;
+                }
+                // one reciprocal square root for the pivot and the row
+                const double rs = rsqrt(piv);
+                double ur[NB];
+#pragma unroll
+                for (int I = K; I < NB; ++I) ur[I] = rowb[16 * I + c] * rs;
+                double ui[4];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const int r = g + 4 * qq;
+                    const double v = rowb[16 * K + r] * rs;
+                    ui[qq] = r > jr ? v : 0.0;
+                }
+                const double urk = c > jr ? ur[K] : 0.0;
+                // the holders keep row jr of U
+                if (g == gj) {
+                    M[K][K][q] = c < jr ? 0.0 : (c == jr ? piv * rs : ur[K]);
+#pragma unroll
+                    for (int I = K + 1; I < NB; ++I) M[K][I][q] = ur[I];
+                }
+                // the rows below jr (the holders' own row has ui = 0)
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    M[K][K][qq] -= ui[qq] * urk;
+#pragma unroll
+                    for (int I = K + 1; I < NB; ++I) M[K][I][qq] -= ui[qq] * ur[I];
+                }
+            }
+        }
+        // L x: block row K's part, U_KI^T x_K
+        {
+            double xk[4];
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) xk[qq] = xs[16 * K + g + 4 * qq];
+#pragma unroll
+            for (int I = K; I < NB; ++I)
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) ypart[I] += M[K][I][qq] * xk[qq];
+        }
+        // block rows of padding stay unit rows
+        if (16 * (K + 1) >= n) return true;
+        return regf_block_row<K + 1, NB>(M, ypart, build, rowb, xs, cnt);
+    }
+}
+
+// y = L x (own beads), L the lower Cholesky factor of the RPY tensor at sm.R.
+// false on a non-positive pivot, taken in column order, its index and value
+// in the counters.
+template <int BLOCK, bool MIXED, int NB>
+__device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB>* rlp, int N,
+                                               double hstar, const double x[3], double y[3]) {
+    static_assert(BLOCK == 64, "one wave per chain");
+    const int n = 3 * N, tid = threadIdx.x;
+    const int c = tid & 15, g = tid >> 4;
+    lds_ptr<double> rowb = as_lds_mut(&rlp->row[0][0]);
+    lds_ptr<double> xs = as_lds_mut(rlp->xs);
+    lds_ptr<double> yp = as_lds_mut(&rlp->yp[0][0]);
+    const RegBuild build{as_lds(sm.R[0]), as_lds(sm.R[1]), as_lds(sm.R[2]), n, c, g, hstar, 1.0 / hstar};
+
+    __syncthreads();
+    if (tid < N) {
+        xs[3 * tid] = x[0];
+        xs[3 * tid + 1] = x[1];
+        xs[3 * tid + 2] = x[2];
+    }
+    for (int e = n + tid; e < 16 * NB; e += 64) xs[e] = 0.0;
+    __syncthreads();
+
+    mfma_d4 M[NB][NB];
+    double ypart[NB];
+#pragma unroll
+    for (int I = 0; I < NB; ++I) ypart[I] = 0.0;
+    if (!regf_block_row<0, NB>(M, ypart, build, rowb, xs, sm.cnt)) return false;
+    __syncthreads();
+#pragma unroll
+    for (int I = 0; I < NB; ++I) yp[g * 16 * NB + 16 * I + c] = ypart[I];
+    __syncthreads();
+    y[0] = y[1] = y[2] = 0.0;
+    if (tid < N)
+        for (int d = 0; d < 3; ++d) {
+            const int e = 3 * tid + d;
+            y[d] = (yp[e] + yp[16 * NB + e]) + (yp[32 * NB + e] + yp[48 * NB + e]);
+        }
+    return true;
+}
+
 // Barrier of the parts of one split chain (thread 0 of each part): count in,
 // then wait until the count reaches `target`. Bounded: false after about a
 // second, so that a part that never arrives cannot hold the others forever.
@@ -617,8 +834,11 @@ __device__ __noinline__ bool chain_barrier(uint32_t* bar, uint32_t target) {
 // coefficients and nodes, inverse-iteration rows, Lanczos basis) lives in
 // LDS (for launches of few chains, whose steps are bound by the latency of
 // these round trips); the dense region stays in HBM.
+// REGF (NB > 0, FT double): the Cholesky method's factor in registers
+// (dense_bdw_regs) instead of LDS; the step draws the noise first and factors
+// after, restoring the generator state when the factor fails.
 template <int BLOCK, bool MIXED, int NB = 0, bool SPLIT = false, typename FT = double,
-          bool LWS = false>
+          bool LWS = false, bool REGF = false>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
 #ifndef BDSIM_MIXED_RITZ_F32
@@ -643,15 +863,19 @@ struct Chain {
     uint32_t epoch = 0;         // SPLIT: exchanges done
     bool sync_fail = false;     // SPLIT: a barrier timed out
     bool dense_ws = false;      // this step's factor went to the workspace
+    using RL = RegLds<(NB > 0 ? NB : 1)>;
+    RL* rl = nullptr;           // REGF: the register factor's LDS buffers
 
     __device__ __forceinline__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
-                     FT* dense_blk = nullptr, double* dense_xs = nullptr, double* lds_ws = nullptr)
+                     FT* dense_blk = nullptr, double* dense_xs = nullptr, double* lds_ws = nullptr,
+                     void* reg_lds = nullptr)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
         wsg = as_gbl(A.ws + P.ws_off);
         if constexpr (LWS) ws = as_lds_mut(lds_ws);
         else ws = wsg;
         dlb = as_lds_mut(dense_blk);
+        rl = static_cast<RL*>(reg_lds);
         dxs = as_lds_mut(dense_xs);
     }
 
@@ -1301,7 +1525,7 @@ struct Chain {
     // workspace, so the FP32 variant fails only where FP64 does.
     __device__ __forceinline__ bool dense_factor() {
         dense_ws = false;
-        if constexpr (NB > 0) {
+        if constexpr (NB > 0 && !REGF) {
             if (n3 <= 16 * NB) {
                 if (dense_factor_lds<BLOCK, MIXED, NB, FT>(sm, dlb, N, P.hstar)) return true;
                 if constexpr (sizeof(FT) == 8) return false;
@@ -1317,7 +1541,7 @@ struct Chain {
     // y = L x with the factor where dense_factor left it, own rows.
     __device__ __forceinline__ void lower_matvec(const double x[3], double y[3]) {
         D3 r;
-        if constexpr (NB > 0) {
+        if constexpr (NB > 0 && !REGF) {
             if (n3 <= 16 * NB && !dense_ws) r = lower_matvec_lds<BLOCK, MIXED, NB, FT>(dlb, dxs, N, x[0], x[1], x[2]);
             else r = lower_matvec_ool<BLOCK, MIXED>(sm, wsg + wl.dense, N, x[0], x[1], x[2]);
         } else {
@@ -1640,11 +1864,15 @@ struct Chain {
             }
 
             // mobility at R; the Cholesky method factors here (mobility.cpp:39-51)
-            if (hi && P.hi_method == 1) {
+            if (!REGF && hi && P.hi_method == 1) {
                 if (!dense_factor()) { status = ST_NOT_PD; break; }
             }
 
-            // B.dW (mobility.cpp:58-88)
+            // B.dW (mobility.cpp:58-88). REGF: the factor follows the draws;
+            // a failed factor leaves the generator where the reference's
+            // (which factors before drawing) would be.
+            const int32_t ix_b = ix, iy_b = iy, rseed_b = rseed;
+            const bool ready_b = rng_ready;
             if (!rng_ready) { rng_maybe_init(ix, iy, rseed); rng_ready = true; }
             double bx0[3] = {0.0, 0.0, 0.0};
             {
@@ -1658,7 +1886,18 @@ struct Chain {
                 if (!hi) {
                     for (int d = 0; d < 3; ++d) bx0[d] = x0[d];
                 } else if (P.hi_method == 1) {
-                    lower_matvec(x0, bx0);
+                    if constexpr (REGF) {
+                        if (!dense_bdw_regs<BLOCK, MIXED, NB>(sm, rl, N, P.hstar, x0, bx0)) {
+                            ix = ix_b;
+                            iy = iy_b;
+                            rseed = rseed_b;
+                            rng_ready = ready_b;
+                            status = ST_NOT_PD;
+                            break;
+                        }
+                    } else {
+                        lower_matvec(x0, bx0);
+                    }
                 } else if (P.hi_method == 0) {
                     save(SLOT_X0, x0);
                 } else {
@@ -1872,6 +2111,25 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
     init_counters(sm);
     if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
     Chain<BLOCK, MIXED, NB, false, FT> c(sm, *P, a, dl.blk, dl.xs);
+    c.run(ti);
+}
+
+// Chains of the Cholesky method with 3N <= 16 NB, the FP64 factor in
+// registers (dense_bdw_regs), 13 KB of LDS per chain. WPE: the waves per SIMD
+// the register allocation is held to.
+template <bool MIXED, int NB, int WPE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_integrate_dreg(KArgs a) {
+    __shared__ Smem<64, MIXED> sm;
+    __shared__ RegLds<NB> rl;
+    const int ti = a.list[blockIdx.x];
+    const TrajParams* __restrict__ P = a.tp + ti;
+    if (a.seg && a.seg[ti].done) {
+        if (threadIdx.x == 0) a.done_out[blockIdx.x] = 1;
+        return;
+    }
+    init_counters(sm);
+    if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
+    Chain<64, MIXED, NB, false, double, false, true> c(sm, *P, a, nullptr, nullptr, nullptr, &rl);
     c.run(ti);
 }
 

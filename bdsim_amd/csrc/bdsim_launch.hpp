@@ -5,6 +5,8 @@
 #pragma once
 #include "bdsim_chain.hpp"
 
+#include <cstdlib>
+
 namespace bdgpu {
 
 // 64-thread chains: k_integrate<64, MIXED, wpe> (wpe = waves per SIMD the
@@ -16,6 +18,21 @@ namespace bdgpu {
 template <bool MIXED>
 void launch_integrate_small(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
                             const KArgs& a, size_t lws_bytes = 0);
+
+// Cholesky chains with the FP64 factor in registers: k_integrate_dreg<MIXED,
+// nb>, nb = 4 or 6 (k_integrate_dreg.hip).
+template <bool MIXED>
+void launch_integrate_dreg(int nb, dim3 grid, hipStream_t stream, const KArgs& a);
+
+// The register factor is the default for FP64 factors; BDSIM_DENSE_REG=0 in
+// the environment keeps them in LDS (k_integrate_dense).
+inline bool dense_reg_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("BDSIM_DENSE_REG");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
 
 // Multi-wave chains: k_integrate<block, MIXED, wpe>, block 128..1024, wpe 1 or
 // 4 (1024: 1).

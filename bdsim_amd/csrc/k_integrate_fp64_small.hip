@@ -1,6 +1,6 @@
 // k_integrate_fp64_small.hip -- the fp64 form's integrate kernels for chains of
 // up to 64 beads (one wave per chain) and the Cholesky kernels with the factor
-// in LDS.
+// in LDS (the register-factor kernels are in k_integrate_dreg.hip).
 #include "bdsim_launch.hpp"
 
 namespace bdgpu {
@@ -10,6 +10,10 @@ void launch_integrate_small<false>(int wpe, int nb, bool, dim3 grid, hipStream_t
                                     const KArgs& a, size_t lws_bytes) {
     if (lws_bytes > 0 && nb == 0) {
         hipLaunchKernelGGL((k_integrate_lws<false>), grid, dim3(64), lws_bytes, stream, a);
+        return;
+    }
+    if ((nb == 4 || nb == 6) && dense_reg_enabled()) {
+        launch_integrate_dreg<false>(nb, grid, stream, a);
         return;
     }
     if (nb == 4) hipLaunchKernelGGL((k_integrate_dense<64, false, 4>), grid, dim3(64), 0, stream, a);
