@@ -673,7 +673,7 @@ struct RegBuild {
 // registers; U_JK dies once row K is formed, so (K + 1)(NB - K) blocks are
 // live at most (12 at NB = 6). false on a non-positive pivot.
 template <int K, int NB>
-__device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], double (&ypart)[NB], const RegBuild& build,
+__device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<double> yp, const RegBuild& build,
                                                lds_ptr<double> rowb, lds_cptr<double> xs, Counters& cnt) {
     if constexpr (K == NB) {
         return true;
@@ -746,19 +746,23 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], double (&yp
                 }
             }
         }
-        // L x: block row K's part, U_KI^T x_K
+        // L x: block row K's part, U_KI^T x_K, added to this lane's partial
+        // sums in LDS (lane 16 g + c owns entries (g, 16 I + c))
         {
             double xk[4];
 #pragma unroll
             for (int qq = 0; qq < 4; ++qq) xk[qq] = xs[16 * K + g + 4 * qq];
 #pragma unroll
-            for (int I = K; I < NB; ++I)
+            for (int I = K; I < NB; ++I) {
+                double s = K == 0 ? 0.0 : yp[g * 16 * NB + 16 * I + c];
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) ypart[I] += M[K][I][qq] * xk[qq];
+                for (int qq = 0; qq < 4; ++qq) s += M[K][I][qq] * xk[qq];
+                yp[g * 16 * NB + 16 * I + c] = s;
+            }
         }
         // block rows of padding stay unit rows
         if (16 * (K + 1) >= n) return true;
-        return regf_block_row<K + 1, NB>(M, ypart, build, rowb, xs, cnt);
+        return regf_block_row<K + 1, NB>(M, yp, build, rowb, xs, cnt);
     }
 }
 
@@ -786,13 +790,7 @@ __device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB
     __syncthreads();
 
     mfma_d4 M[NB][NB];
-    double ypart[NB];
-#pragma unroll
-    for (int I = 0; I < NB; ++I) ypart[I] = 0.0;
-    if (!regf_block_row<0, NB>(M, ypart, build, rowb, xs, sm.cnt)) return false;
-    __syncthreads();
-#pragma unroll
-    for (int I = 0; I < NB; ++I) yp[g * 16 * NB + 16 * I + c] = ypart[I];
+    if (!regf_block_row<0, NB>(M, yp, build, rowb, xs, sm.cnt)) return false;
     __syncthreads();
     y[0] = y[1] = y[2] = 0.0;
     if (tid < N)
