@@ -695,13 +695,16 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<dou
             }
         }
         // rows jr = 4 q + gj of block row K, in order (rows of padding stay
-        // unit rows); row jr is held by the lanes of row group gj, value q
+        // unit rows); row jr is held by the lanes of row group gj, value q.
+        // With q and gj unrolled, only the rows of value q depend on the
+        // lane's row group: values qq < q are finished rows, qq > q rows below.
         const int rows = min(16, n - 16 * K);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int ng = min(4, rows - 4 * q);
-            for (int gj = 0; gj < ng; ++gj) {
+#pragma unroll
+            for (int gj = 0; gj < 4; ++gj) {
                 const int jr = 4 * q + gj;
+                if (jr >= rows) break;
                 if (g == gj) {
 #pragma unroll
                     for (int I = K; I < NB; ++I) rowb[16 * I + c] = M[K][I][q];
@@ -723,29 +726,32 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<dou
                 double ur[NB];
 #pragma unroll
                 for (int I = K; I < NB; ++I) ur[I] = rowb[16 * I + c] * rs;
-                double ui[4];
+                // the holders scale row jr (its pivot becomes piv rs; its
+                // entries left of the pivot are cleared after the block row)
+                const double f = g == gj ? rs : 1.0;
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) {
-                    const int r = g + 4 * qq;
-                    const double v = rowb[16 * K + r] * rs;
-                    ui[qq] = r > jr ? v : 0.0;
-                }
+                for (int I = K; I < NB; ++I) M[K][I][q] *= f;
+                // rank-1 update of the rows below jr, right of the pivot
                 const double urk = c > jr ? ur[K] : 0.0;
-                // the holders keep row jr of U
-                if (g == gj) {
-                    M[K][K][q] = c < jr ? 0.0 : (c == jr ? piv * rs : ur[K]);
+                {
+                    const double v = rowb[16 * K + g + 4 * q] * rs;
+                    const double u = g > gj ? v : 0.0;
+                    M[K][K][q] -= u * urk;
 #pragma unroll
-                    for (int I = K + 1; I < NB; ++I) M[K][I][q] = ur[I];
+                    for (int I = K + 1; I < NB; ++I) M[K][I][q] -= u * ur[I];
                 }
-                // the rows below jr (the holders' own row has ui = 0)
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) {
-                    M[K][K][qq] -= ui[qq] * urk;
+                for (int qq = q + 1; qq < 4; ++qq) {
+                    const double u = rowb[16 * K + g + 4 * qq] * rs;
+                    M[K][K][qq] -= u * urk;
 #pragma unroll
-                    for (int I = K + 1; I < NB; ++I) M[K][I][qq] -= ui[qq] * ur[I];
+                    for (int I = K + 1; I < NB; ++I) M[K][I][qq] -= u * ur[I];
                 }
             }
         }
+        // U_KK is upper triangular
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) M[K][K][qq] = c < g + 4 * qq ? 0.0 : M[K][K][qq];
         // L x: block row K's part, U_KI^T x_K, added to this lane's partial
         // sums in LDS (lane 16 g + c owns entries (g, 16 I + c))
         {
@@ -1896,6 +1902,11 @@ struct Chain {
                     } else {
                         lower_matvec(x0, bx0);
                     }
+                } else if (REGF) {
+                    // the register-factor kernel is built for the Cholesky
+                    // method alone (its launcher sends it nothing else)
+                    status = ST_NOT_RUN;
+                    break;
                 } else if (P.hi_method == 0) {
                     save(SLOT_X0, x0);
                 } else {
@@ -1903,8 +1914,10 @@ struct Chain {
                     break;
                 }
             }
-            if (hi && P.hi_method == 0) {
-                if (!chebyshev_bx0(seed_valid, bx0)) { status = ST_NOT_PD; break; }
+            if constexpr (!REGF) {
+                if (hi && P.hi_method == 0) {
+                    if (!chebyshev_bx0(seed_valid, bx0)) { status = ST_NOT_PD; break; }
+                }
             }
             {
                 const double sqrt_dt = sqrt(dt);

@@ -71,7 +71,7 @@ struct RngState { int32_t ix, iy, seed; };
 struct SegState { double t; int32_t seed_valid, done; };
 
 enum : int32_t { ST_OK = 0, ST_NOT_PD = 1, ST_RTSAFE_UNBRACKETED = 2, ST_RTSAFE_MAXIT = 3,
-                 ST_EXACT_SQRT = 4, ST_WLCB_SIGMA = 5, ST_SYNC_TIMEOUT = 7 };
+                 ST_EXACT_SQRT = 4, ST_WLCB_SIGMA = 5, ST_NOT_RUN = 6, ST_SYNC_TIMEOUT = 7 };
 
 // Per-trajectory workspace in HBM (doubles). The state slots hold per-bead
 // vectors that live across phases of a step (structure of arrays: slot k of
