@@ -111,7 +111,7 @@ struct SrvJob {
 
 struct SrvGroup {
     int block = 64;
-    int nb = 0;                       // dense LDS kernel (k_integrate_dense<64, *, nb>)
+    int nb = 0;                       // Cholesky kernel of 16 nb rows (k_integrate_dreg<*, nb, 2>)
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     bool in_flight = false;
