@@ -1206,7 +1206,8 @@ struct Chain {
     // it adds. FP32 runs of at most 16 steps are folded into FP64 sums.
     // y = x + the sum (identity diagonal blocks). A chain split over parts
     // (SPLIT): part p walks the tile pairs (w, t) with t = p (mod parts), own
-    // tiles included, and y is its share of the sum alone (no x).
+    // tiles included, and y is its share of the sum alone (no x). A workgroup with
+    // more waves than tiles spreads the walks over all its waves (see below).
     __device__ __forceinline__ void apply_tiles(const double v[3], double y[3]) {
         int e = 0;
 #pragma unroll
@@ -1218,21 +1219,35 @@ struct Chain {
         W.init(P.rpy_f);
         const int w = tid >> 6, l = tid & 63;
         const int T = (N + 63) >> 6;
+        // more waves than tiles (a chain on a wider workgroup than its beads
+        // need): the T^2 walks of 32 pair steps (own tile w; the half of the
+        // cross pair (w, t) walked from w) go round robin to all waves, and
+        // every row's sums, own rows included, go through Acc -- where that
+        // shortens the longest wave's share (ceil(T^2 / NW) < T; built for
+        // workgroups of 8 waves or more, where that can happen by default)
+        constexpr bool CAN_SPREAD = !SPLIT && S::NW >= 8;
+        const bool spread = CAN_SPREAD && T * T <= S::NW * (T - 1);
         const int rot = 4 * ((l + 63) & 63);            // take from lane l - 1
         auto perm = [](int addr, float x) {
             return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
         };
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        if (w < T) {
-            const lds_cptr<f32x4> pa0 = as_lds(&sm.Pa[0]);
-            const lds_cptr<f32x2> pb0 = as_lds(&sm.Pb[0]);
-            const f32x4 mine = pa0[128 * w + l];
-            const f32x2 mine2 = pb0[128 * w + l];
+        const lds_cptr<f32x4> pa0 = as_lds(&sm.Pa[0]);
+        const lds_cptr<f32x2> pb0 = as_lds(&sm.Pb[0]);
+        // LDS-qualified, so that these are ds_add_u64 (not flat atomics)
+        const lds_ptr<unsigned long long> acc0 =
+            as_lds_mut(reinterpret_cast<unsigned long long*>(&sm.Acc[0][0]));
+        // one walk: this lane's bead of tile tw against tile tt (tt == tw:
+        // the own tile), own-row sums into (ox, oy, oz); OWN: 1 own tile only,
+        // 0 other tiles only, -1 either
+        auto walk_t = [&](auto own_sel, int tw, int tt, double& ox, double& oy, double& oz) {
+            constexpr int OWN = decltype(own_sel)::value;
+            const f32x4 mine = pa0[128 * tw + l];
+            const f32x2 mine2 = pb0[128 * tw + l];
             const float rx = mine.x, ry = mine.y, rz = mine.z;
             const float xi = mine.w, yi = mine2.x, zi = mine2.y;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            lds_cptr<f32x4> pa = pa0 + 128 * w + l;
-            lds_cptr<f32x2> pb = pb0 + 128 * w + l;
+            const lds_cptr<f32x4> pa = pa0 + 128 * tt + l;
+            const lds_cptr<f32x2> pb = pb0 + 128 * tt + l;
             auto pair = [&](int k, bool both, float& fx, float& fy, float& fz) {
                 const f32x4 q = pa[k];
                 const f32x2 q2 = pb[k];
@@ -1252,18 +1267,18 @@ struct Chain {
                     sx = perm(rot, sx); sy = perm(rot, sy); sz = perm(rot, sz);
                 }
             };
-            // (1) own tile: after step kend the rotated terms are for row
-            // l + kend - 1 ... brought to their lane by one permute
-            if (!SPLIT || w % nparts == part) {
+            if (OWN == 1 || (OWN == -1 && tt == tw)) {
+                // (1) own tile: after step kend the rotated terms are for row
+                // l + kend - 1 ... brought to their lane by one permute
                 float fx = 0.0f, fy = 0.0f, fz = 0.0f;
                 pair(32, false, fx, fy, fz);
-                ax += (double)fx; ay += (double)fy; az += (double)fz;
+                ox += (double)fx; oy += (double)fy; oz += (double)fz;
                 auto flush = [&](int kend, float gx, float gy, float gz) {
-                    ax += (double)gx; ay += (double)gy; az += (double)gz;
+                    ox += (double)gx; oy += (double)gy; oz += (double)gz;
                     const int from = 4 * ((l - kend) & 63);
-                    ax += (double)perm(from, sx);
-                    ay += (double)perm(from, sy);
-                    az += (double)perm(from, sz);
+                    ox += (double)perm(from, sx);
+                    oy += (double)perm(from, sy);
+                    oz += (double)perm(from, sz);
                     sx = sy = sz = 0.0f;
                 };
                 int k = 31;
@@ -1278,26 +1293,19 @@ struct Chain {
                     for (; k > 0; --k) pair(k, true, gx, gy, gz);
                     flush(0, gx, gy, gz);
                 }
-            }
-            // (2) the other tiles: 32 offsets each, in two runs of 16; after a
-            // run ending at offset kend, lane j's rotated sums are for row
-            // (j - 1 + kend) mod 64 of tile t
-            for (int t = SPLIT ? part : 0; t < T; t += SPLIT ? nparts : 1) {
-                if (t == w) continue;
-                pa = pa0 + 128 * t + l;
-                pb = pb0 + 128 * t + l;
-                const int khi = w < t ? 63 : 64;
+            } else {
+                // (2) another tile: 32 offsets, in two runs of 16; after a run
+                // ending at offset kend, lane j's rotated sums are for row
+                // (j - 1 + kend) mod 64 of tile tt
+                const int khi = tw < tt ? 63 : 64;
 #pragma unroll 1
                 for (int r = 0; r < 2; ++r) {
                     const int k0 = khi - 16 * r;
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll 4
                     for (int u = 0; u < 16; ++u) pair(k0 - u, true, gx, gy, gz);
-                    ax += (double)gx; ay += (double)gy; az += (double)gz;
-                    const int row = 64 * t + ((l + k0 - 16) & 63);
-                    // LDS-qualified, so that these are ds_add_u64 (not flat atomics)
-                    const lds_ptr<unsigned long long> acc =
-                        as_lds_mut(reinterpret_cast<unsigned long long*>(&sm.Acc[0][0])) + row;
+                    ox += (double)gx; oy += (double)gy; oz += (double)gz;
+                    const lds_ptr<unsigned long long> acc = acc0 + 64 * tt + ((l + k0 - 16) & 63);
                     __hip_atomic_fetch_add(acc, (unsigned long long)(long long)__float2ll_rn(sx * FIX),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     __hip_atomic_fetch_add(acc + S::AB, (unsigned long long)(long long)__float2ll_rn(sy * FIX),
@@ -1306,6 +1314,41 @@ struct Chain {
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     sx = sy = sz = 0.0f;
                 }
+            }
+        };
+        auto walk = [&](int tw, int tt, double& ox, double& oy, double& oz) {
+            walk_t(std::integral_constant<int, -1>{}, tw, tt, ox, oy, oz);
+        };
+        auto walk_own = [&](int tw, double& ox, double& oy, double& oz) {
+            walk_t(std::integral_constant<int, 1>{}, tw, tw, ox, oy, oz);
+        };
+        auto walk_cross = [&](int tw, int tt, double& ox, double& oy, double& oz) {
+            walk_t(std::integral_constant<int, 0>{}, tw, tt, ox, oy, oz);
+        };
+        // this wave's walks: spread, units u = w, w + NW, ... as (u / T, u mod
+        // T), each unit's own-row sums flushed to Acc; otherwise (wave w < T)
+        // its own tile, then the other tiles (SPLIT: the tiles t = part (mod
+        // parts), own tile included)
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (spread) {
+            const double FIXd = (double)FIX;
+            for (int u = w; u < T * T; u += S::NW) {
+                const int tw = u / T, tt = u - T * tw;
+                double ox = 0.0, oy = 0.0, oz = 0.0;
+                walk(tw, tt, ox, oy, oz);
+                const lds_ptr<unsigned long long> acc = acc0 + 64 * tw + l;
+                __hip_atomic_fetch_add(acc, (unsigned long long)(long long)__double2ll_rn(ox * FIXd),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(acc + S::AB, (unsigned long long)(long long)__double2ll_rn(oy * FIXd),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(acc + 2 * S::AB, (unsigned long long)(long long)__double2ll_rn(oz * FIXd),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        } else if (w < T) {
+            if (!SPLIT || w % nparts == part) walk_own(w, ax, ay, az);
+            for (int t = SPLIT ? part : 0; t < T; t += SPLIT ? nparts : 1) {
+                if (t == w) continue;
+                walk_cross(w, t, ax, ay, az);
             }
         }
         __syncthreads();

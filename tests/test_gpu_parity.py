@@ -356,6 +356,39 @@ def test_split_chains(gpu, orc, n, N):
         assert rel(mixed[i], want) < 1e-5, (i, rel(mixed[i], want))
 
 
+def test_wide_workgroups_spread_the_pair_walk(gpu, orc):
+    """Chains on workgroups with more waves than 64-bead tiles (N=329: 6 tiles
+    on 8 waves; N=700: 11 tiles on 16, not split because the batch holds
+    shorter chains too) spread the mixed form's tile walks over all waves, every
+    row through the fixed-point sums: 20 steps within 1e-5 of the FP64 oracle
+    (FP64 form within 1e-10 with the oracle's decisions), and a second run
+    equal to the first bit for bit (the integer sums do not depend on the order
+    in which the waves add)."""
+    d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=1, zstar=0.3, dstar=0.5,
+                    kappa=C.wi_kappa(0.05))
+    sim = H.csim(C.sim_dict(0.0, 0.019, 1e-3, 1e-3))
+    items, want = [], []
+    for i, N in enumerate((329, 700, 329, 700)):
+        p = H.Packed(d, N)
+        R0 = W.self_avoiding_walk(N, np.random.default_rng(90 + i))
+        items.append((N, R0, p.c, sim, H.rng_of(i)))
+        if i < 2:
+            st = H.rng_of(i)
+            w, rc, cnto, lo = H.oracle_integrate(orc, N, R0, p.c, sim, st)
+            assert rc == 0
+            want.append(w)
+    out, _, status, _, _ = H.gpu_batch(gpu, items)
+    assert np.all(status == 0)
+    m1, _, st1, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    m2, _, st2, _, _ = H.gpu_batch(gpu, items, precision=A.PRECISION_MIXED)
+    assert np.all(st1 == 0) and np.all(st2 == 0)
+    for i in range(2):
+        assert rel(out[i], want[i]) <= 1e-10, (i, rel(out[i], want[i]))
+        assert rel(m1[i], want[i]) < 1e-5, (i, rel(m1[i], want[i]))
+    for a, b in zip(m1, m2):
+        assert np.array_equal(a, b)
+
+
 def test_resident_split_chains_match(gpu):
     """A resident batch of 3 chains of N=600 (split over 8 workgroups each)
     over two segments equals bdsim_integrate_batch on the same chains (split
