@@ -229,6 +229,8 @@ def _declare_gpu(lib):
     lib.bdsim_tune_get.argtypes = [c_int]
     lib.bdsim_tune_split.argtypes = [c_int]
     lib.bdsim_tune_split_get.argtypes = []
+    lib.bdsim_tune_width.argtypes = [c_int, c_int]
+    lib.bdsim_tune_width_get.argtypes = [c_int]
     lib.bdsim_integrate_batch.argtypes = [c_int, _ip, _dp, P(CPhys), P(CSim), P(CRng), c_int,
                                           _ip, P(CCounters), P(CStepLog), c_int]
     lib.bdsim_integrate_batch_static.argtypes = lib.bdsim_integrate_batch.argtypes

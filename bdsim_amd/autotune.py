@@ -7,16 +7,20 @@ exists in builds for different register budgets (waves per SIMD: 1, 2 or 4 for
 spills, fewer means more chains resident at once. Chains of more than 512
 beads, when they alone leave compute units idle, run split over several
 workgroups each (waves per chain: 16 per workgroup times G workgroups); more
-parts means less work per part but more exchanges to wait for. Which wins
-depends on the mix, so `tune` times the candidates on a representative batch
--- one bucket at a time, the others on the library's own choice, all buckets
-running side by side as they do in use -- and keeps the fastest, and for
-batches of long chains the largest G. The table ({block: waves per SIMD},
-and under key 1024 the workgroups per long chain) applies with `apply`
-(bdsim_tune_set, bdsim_tune_split) and round-trips through JSON (`save` /
+parts means less work per part but more exchanges to wait for. In the mixed
+form the chains of the 128..512 buckets may also run on wider workgroups
+(block x m threads, m = 1, 2, 4, 8 up to 1024: more waves per chain, the tiled
+pair walk spread over them), which shortens a latency-bound step but holds
+more waves per chain. Which wins depends on the mix, so `tune` times the
+candidates on a representative batch -- one bucket at a time, the others on
+the library's own choice, all buckets running side by side as they do in use
+-- and keeps the fastest, and for batches of long chains the largest G. The
+table ({block: waves per SIMD}, under key 1024 the workgroups per long chain,
+under "w<block>" the width multiplier) applies with `apply` (bdsim_tune_set,
+bdsim_tune_split, bdsim_tune_width) and round-trips through JSON (`save` /
 `load`). Register budgets change time only (the same operations, another
-register allocation); G also changes the order in which partner sums are
-added.
+register allocation); G and the width also change the order in which partner
+sums are added (the width: exactly, through fixed-point sums).
 """
 import json
 import time
@@ -28,6 +32,12 @@ from . import _abi
 CANDIDATES = {64: (1, 2, 4), 128: (1, 4), 256: (1, 4), 512: (1, 4)}
 SPLIT_KEY = 1024                      # table key of the workgroups per long chain
 SPLIT_CANDIDATES = (1, 2, 4, 8, 16)
+WIDTH_CANDIDATES = {128: (1, 2, 4, 8), 256: (1, 2, 4), 512: (1, 2)}
+
+
+def width_key(block):
+    """Table key of the workgroup width multiplier of a block size."""
+    return f"w{int(block)}"
 
 
 def block_of(N):
@@ -39,24 +49,30 @@ def block_of(N):
 
 
 def apply(table, lib=None):
-    """Set the library's budgets from {block: waves} and the workgroups per
-    long chain from table[1024] (0 = automatic)."""
+    """Set the library's budgets from {block: waves}, the workgroups per long
+    chain from table[1024] and the width multipliers from table["w<block>"]
+    (0 = automatic)."""
     lib = lib or _abi.load_gpu()
-    for block, w in table.items():
-        if int(block) == SPLIT_KEY:
-            _abi.check(lib, lib.bdsim_tune_split(int(w)))
+    for key, v in table.items():
+        if isinstance(key, str):
+            _abi.check(lib, lib.bdsim_tune_width(int(key[1:]), int(v)))
+        elif int(key) == SPLIT_KEY:
+            _abi.check(lib, lib.bdsim_tune_split(int(v)))
         else:
-            _abi.check(lib, lib.bdsim_tune_set(int(block), int(w)))
+            _abi.check(lib, lib.bdsim_tune_set(int(key), int(v)))
 
 
 def reset(lib=None):
-    apply({**{b: 0 for b in CANDIDATES}, SPLIT_KEY: 0}, lib)
+    apply({**{b: 0 for b in CANDIDATES}, SPLIT_KEY: 0, **{width_key(b): 0 for b in WIDTH_CANDIDATES}},
+          lib)
 
 
 def current(lib=None):
     lib = lib or _abi.load_gpu()
     out = {b: int(lib.bdsim_tune_get(b)) for b in CANDIDATES}
     out[SPLIT_KEY] = int(lib.bdsim_tune_split_get())
+    for b in WIDTH_CANDIDATES:
+        out[width_key(b)] = int(lib.bdsim_tune_width_get(b))
     return out
 
 
@@ -67,7 +83,7 @@ def save(table, path):
 
 def load(path):
     with open(path) as f:
-        return {int(k): int(v) for k, v in json.load(f).items()}
+        return {(int(k) if k.isdigit() else k): int(v) for k, v in json.load(f).items()}
 
 
 def _timed(lib, N, R, phys, dt, precision, steps, warmup):
@@ -93,10 +109,11 @@ def _timed(lib, N, R, phys, dt, precision, steps, warmup):
 def tune(N, R, phys, dt, precision=_abi.PRECISION_MIXED, steps=20, warmup=3, repeats=2,
          lib=None, verbose=False):
     """Fastest budget per bucket present in the batch (N: chain lengths, R:
-    their positions, phys: CPhys per chain, dt) and, for a batch of chains of
-    more than 512 beads only, the fastest number of workgroups per chain.
-    Returns {block: waves} (with 1024: workgroups per chain); the library is
-    left on that table."""
+    their positions, phys: CPhys per chain, dt), for a batch of chains of
+    more than 512 beads only the fastest number of workgroups per chain, and
+    (mixed form) the fastest width per bucket of 128..512 threads. Returns
+    {block: waves} (with 1024: workgroups per chain; "w<block>": width
+    multiplier); the library is left on that table."""
     lib = lib or _abi.load_gpu()
     blocks = sorted({block_of(n) for n in N if block_of(n) in CANDIDATES})
     table = {}
@@ -125,4 +142,18 @@ def tune(N, R, phys, dt, precision=_abi.PRECISION_MIXED, steps=20, warmup=3, rep
                 best = (w, t)
         table[b] = best[0]
         apply({b: best[0]}, lib)
+    if precision != _abi.PRECISION_FP64:
+        for b in blocks:
+            if b not in WIDTH_CANDIDATES:
+                continue
+            best = None
+            for m in WIDTH_CANDIDATES[b]:
+                apply({width_key(b): m}, lib)
+                t = min(_timed(lib, N, R, phys, dt, precision, steps, warmup) for _ in range(repeats))
+                if verbose:
+                    print(f"autotune: block {b} width x{m}: {t * 1e3 / steps:.3f} ms/step")
+                if best is None or t < best[1]:
+                    best = (m, t)
+            table[width_key(b)] = best[0]
+            apply({width_key(b): best[0]}, lib)
     return table

@@ -1203,11 +1203,14 @@ struct Chain {
     // om1 <= 1, om2 |b|^2 < 0.21), so a row's sum over N <= 1024 partners
     // stays below 2^(1+10+e-126) units of 2^-ks, i.e. 2^55, and the unit is at
     // most 2^-43 of the largest entry, far below the FP32 rounding of the runs
-    // it adds. FP32 runs of at most 16 steps are folded into FP64 sums.
+    // it adds. FP32 runs of at most 16 steps are folded into FP64 sums, and
+    // each walk's own-row sums go to Acc in fixed point as well, so every row
+    // is one exact integer sum whichever wave walked each part.
     // y = x + the sum (identity diagonal blocks). A chain split over parts
     // (SPLIT): part p walks the tile pairs (w, t) with t = p (mod parts), own
-    // tiles included, and y is its share of the sum alone (no x). A workgroup with
-    // more waves than tiles spreads the walks over all its waves (see below).
+    // tiles included, and y is its share of the sum alone (no x). A workgroup
+    // with more waves than tiles spreads the walks over all its waves, with
+    // the same result bit for bit.
     __device__ __forceinline__ void apply_tiles(const double v[3], double y[3]) {
         int e = 0;
 #pragma unroll
@@ -1325,38 +1328,48 @@ struct Chain {
         auto walk_cross = [&](int tw, int tt, double& ox, double& oy, double& oz) {
             walk_t(std::integral_constant<int, 0>{}, tw, tt, ox, oy, oz);
         };
+        // Every walk's own-row sums go to Acc in fixed point too, so a row's
+        // sum is the same integer whichever wave walked each part: the result
+        // does not depend on the workgroup's width (or on the order of waves).
+        const double FIXd = (double)FIX;
+        auto flush_own = [&](int tw, double ox, double oy, double oz) {
+            const lds_ptr<unsigned long long> acc = acc0 + 64 * tw + l;
+            __hip_atomic_fetch_add(acc, (unsigned long long)(long long)__double2ll_rn(ox * FIXd),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc + S::AB, (unsigned long long)(long long)__double2ll_rn(oy * FIXd),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc + 2 * S::AB, (unsigned long long)(long long)__double2ll_rn(oz * FIXd),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
         // this wave's walks: spread, units u = w, w + NW, ... as (u / T, u mod
-        // T), each unit's own-row sums flushed to Acc; otherwise (wave w < T)
-        // its own tile, then the other tiles (SPLIT: the tiles t = part (mod
-        // parts), own tile included)
-        double ax = 0.0, ay = 0.0, az = 0.0;
+        // T); otherwise (wave w < T) its own tile, then the other tiles
+        // (SPLIT: the tiles t = part (mod parts), own tile included)
         if (spread) {
-            const double FIXd = (double)FIX;
             for (int u = w; u < T * T; u += S::NW) {
                 const int tw = u / T, tt = u - T * tw;
                 double ox = 0.0, oy = 0.0, oz = 0.0;
                 walk(tw, tt, ox, oy, oz);
-                const lds_ptr<unsigned long long> acc = acc0 + 64 * tw + l;
-                __hip_atomic_fetch_add(acc, (unsigned long long)(long long)__double2ll_rn(ox * FIXd),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(acc + S::AB, (unsigned long long)(long long)__double2ll_rn(oy * FIXd),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(acc + 2 * S::AB, (unsigned long long)(long long)__double2ll_rn(oz * FIXd),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                flush_own(tw, ox, oy, oz);
             }
         } else if (w < T) {
-            if (!SPLIT || w % nparts == part) walk_own(w, ax, ay, az);
+            if (!SPLIT || w % nparts == part) {
+                double ox = 0.0, oy = 0.0, oz = 0.0;
+                walk_own(w, ox, oy, oz);
+                flush_own(w, ox, oy, oz);
+            }
             for (int t = SPLIT ? part : 0; t < T; t += SPLIT ? nparts : 1) {
                 if (t == w) continue;
-                walk_cross(w, t, ax, ay, az);
+                double ox = 0.0, oy = 0.0, oz = 0.0;
+                walk_cross(w, t, ox, oy, oz);
+                flush_own(w, ox, oy, oz);
             }
         }
         __syncthreads();
         if (own) {
             const double x0 = SPLIT ? 0.0 : v[0], x1 = SPLIT ? 0.0 : v[1], x2 = SPLIT ? 0.0 : v[2];
-            y[0] = x0 + (ax + (double)sm.Acc[0][tid] * inv);
-            y[1] = x1 + (ay + (double)sm.Acc[1][tid] * inv);
-            y[2] = x2 + (az + (double)sm.Acc[2][tid] * inv);
+            y[0] = x0 + (double)sm.Acc[0][tid] * inv;
+            y[1] = x1 + (double)sm.Acc[1][tid] * inv;
+            y[2] = x2 + (double)sm.Acc[2][tid] * inv;
         } else {
             y[0] = y[1] = y[2] = 0.0;
         }

@@ -330,6 +330,39 @@ size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
     return (size_t)n_wg <= (size_t)g.n_cu * per_cu ? dyn : 0;
 }
 
+// Workgroup width per block size set by the caller (bdsim_tune_width): the
+// chains of that block run on workgroups of block x m threads (at most 1024),
+// 0 = the library's choice. Index as g_tune.
+std::atomic<int> g_width[4] = {{0}, {0}, {0}, {0}};
+
+// Workgroup width (threads) for the mixed form's multi-wave chains of a grid of
+// `chains` chains of at most nmax beads (0: block's own), launched with
+// `waves` waves in all: wider while the wider workgroup shortens the longest
+// wave's share of the tiled pair walk (T^2 <= NW (T - 1), T tiles, NW >= 8
+// waves: the walk spreads over all waves) and the whole launch still holds no
+// more than one wave per SIMD of the device (a latency-bound launch: the extra
+// waves land on idle SIMDs). BDSIM_WIDE=0 in the environment keeps block.
+int width_choice(int block, int nmax, unsigned chains, long long waves) {
+    static const bool enabled = [] {
+        const char* e = std::getenv("BDSIM_WIDE");
+        return !(e && e[0] == '0');
+    }();
+    const int ti = tune_index(block);
+    const int m = ti >= 0 ? g_width[ti].load() : 0;
+    if (m > 0) return std::min(1024, block * m);
+    if (!enabled || block < 128) return block;
+    const int T = ((nmax > 0 ? nmax : block) + 63) / 64;
+    int wide = block;
+    while (wide < 1024) {
+        const int nw = 2 * wide / 64;
+        const long long more = (long long)chains * (wide / 64);
+        if (!(nw >= 8 && T * T <= nw * (T - 1)) || waves + more > device_simds()) break;
+        waves += more;
+        wide *= 2;
+    }
+    return wide;
+}
+
 // waves: the waves of everything launched together with this grid (buckets
 // side by side), or -1 for this grid alone.
 // f32_factor: the mixed form's variant with the LDS Cholesky factor in FP32.
@@ -352,9 +385,12 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     // Multi-wave chains: the register budget of 4 waves per SIMD when the
     // launch has more waves than 3 per SIMD could hold (all chains then stay
     // resident; measured 2x on 4096 chains of N in [10, 256]), otherwise the
-    // unconstrained build; 1024-thread chains always the latter.
+    // unconstrained build; 1024-thread chains always the latter. The mixed
+    // form may run them on wider workgroups (width_choice).
+    const int wide = MIXED ? width_choice(block, nmax, grid.x, waves) : block;
+    waves += (long long)grid.x * ((wide - block) / 64);
     const bool four = tuned ? tuned == 4 : crowded(waves);
-    launch_integrate_large<MIXED>(block, four && block < 1024 ? 4 : 1, grid, stream, a);
+    launch_integrate_large<MIXED>(wide, four && wide < 1024 ? 4 : 1, grid, stream, a);
 }
 
 // Largest number of workgroups per split chain set by the caller
@@ -704,6 +740,20 @@ int bdsim_tune_set(int block, int waves_per_simd) {
 int bdsim_tune_get(int block) {
     const int i = tune_index(block);
     return i < 0 ? -1 : g_tune[i].load();
+}
+
+int bdsim_tune_width(int block, int multiplier) {
+    const int i = tune_index(block);
+    if (i < 1) return fail(BDSIM_EINVAL, "tune: width is set for blocks 128, 256 or 512");
+    if (!(multiplier == 0 || multiplier == 1 || multiplier == 2 || multiplier == 4 || multiplier == 8))
+        return fail(BDSIM_EINVAL, "tune: width multiplier must be 0 (automatic), 1, 2, 4 or 8");
+    g_width[i].store(multiplier);
+    return 0;
+}
+
+int bdsim_tune_width_get(int block) {
+    const int i = tune_index(block);
+    return i < 1 ? -1 : g_width[i].load();
 }
 
 int bdsim_tune_split(int max_workgroups) {

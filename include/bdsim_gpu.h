@@ -207,6 +207,16 @@ int bdsim_tune_get(int block);           /* current setting, -1 for a bad block 
  * partners). Set by bdsim_amd.autotune; no counterpart in the reference.     */
 int bdsim_tune_split(int max_workgroups);
 int bdsim_tune_split_get(void);
+/* Workgroup width for the mixed form's chains of block size 128, 256 or 512
+ * (chains of 65..512 beads): they run on workgroups of block x multiplier
+ * threads (1, 2, 4 or 8; at most 1024), the tiled pair walk spread over the
+ * extra waves; 0 restores the library's choice (wider while that shortens the
+ * walk and the launch stays within one wave per SIMD). The FP64 form keeps
+ * its block. Results do not depend on it beyond the fixed-point rounding of
+ * the pair sums (2^-43 of the operand). Set by bdsim_amd.autotune; no
+ * counterpart in the reference.                                             */
+int bdsim_tune_width(int block, int multiplier);
+int bdsim_tune_width_get(int block);    /* current setting, -1 for a bad block */
 
 /* ---- the step: batched time_integrate_chain ---------------------------- */
 

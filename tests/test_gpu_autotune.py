@@ -27,19 +27,28 @@ def test_tune_table_and_identical_results(gpu, tmp_path):
     try:
         base, base_rng = _run(gpu, N, R, phys, dt)
         table = autotune.tune(N, R, phys, dt, steps=4, warmup=1, repeats=1, lib=gpu)
-        assert set(table) == {autotune.block_of(n) for n in N} == {64, 128, 256}
+        blocks = {k for k in table if isinstance(k, int)}
+        assert blocks == {autotune.block_of(n) for n in N} == {64, 128, 256}
+        assert {k for k in table if isinstance(k, str)} == {autotune.width_key(128), autotune.width_key(256)}
         for b, w in table.items():
-            assert w in autotune.CANDIDATES[b] and autotune.current(gpu)[b] == w
+            cand = autotune.CANDIDATES[b] if isinstance(b, int) else autotune.WIDTH_CANDIDATES[int(b[1:])]
+            assert w in cand and autotune.current(gpu)[b] == w
         p = tmp_path / "tune.json"
         autotune.save(table, p)
         assert autotune.load(p) == table
-        # every budget gives the same numbers
+        # every budget and every width gives the same numbers
         for b in (64, 128, 256):
             for w in autotune.CANDIDATES[b]:
                 autotune.reset(gpu)
                 autotune.apply({b: w}, gpu)
                 out, rng = _run(gpu, N, R, phys, dt)
                 assert np.array_equal(out, base) and rng == base_rng, (b, w)
+        for b in (128, 256):
+            for m in autotune.WIDTH_CANDIDATES[b]:
+                autotune.reset(gpu)
+                autotune.apply({autotune.width_key(b): m}, gpu)
+                out, rng = _run(gpu, N, R, phys, dt)
+                assert np.array_equal(out, base) and rng == base_rng, (b, m)
     finally:
         autotune.reset(gpu)
     assert all(v == 0 for v in autotune.current(gpu).values())
@@ -50,6 +59,9 @@ def test_tune_rejects_bad_settings(gpu):
     assert gpu.bdsim_tune_set(128, 2) != 0
     assert gpu.bdsim_tune_set(64, 3) != 0
     assert gpu.bdsim_tune_get(96) == -1
+    assert gpu.bdsim_tune_width(64, 2) != 0
+    assert gpu.bdsim_tune_width(256, 3) != 0
+    assert gpu.bdsim_tune_width_get(64) == -1
 
 
 def test_tune_workgroups_per_long_chain(gpu, orc):
