@@ -287,6 +287,12 @@ def load_gpu():
                                  f"{lib.bdsim_last_error().decode()}")
         _gpu = lib
         atexit.register(shutdown)
+        # a launch-shape table saved by bdsim_amd.autotune applies to every
+        # call of this process when BDSIM_TUNE_FILE names it
+        path = os.environ.get("BDSIM_TUNE_FILE")
+        if path:
+            from . import autotune
+            autotune.apply(autotune.load(path), lib)
     return _gpu
 
 
