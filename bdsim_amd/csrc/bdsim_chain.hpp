@@ -2102,7 +2102,7 @@ struct Chain {
             if (resume) {
                 const int32_t done = (status != ST_OK || !(t <= t1 + dt / 2.0)) ? 1 : 0;
                 A.seg[ti] = SegState{t, seed_valid ? 1 : 0, done};
-                A.done_out[blockIdx.x] = done;
+                A.done_out[blockIdx.x / nparts] = done;
             }
         }
     }
@@ -2202,13 +2202,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 
 // Chains of N > 512 split over a.split workgroups each (few chains, many
 // CUs). Launched cooperatively, so that all parts are resident together.
+// Resumable like k_integrate (a.seg): every part reads the chain's segment
+// state at its start (before the first exchange, which part 0 must pass before
+// it writes the state back) and part 0 reports the chain's done flag.
 template <bool MIXED>
 __global__ void __launch_bounds__(1024) k_integrate_split(KArgs a) {
     __shared__ Smem<1024, MIXED> sm;
     const int G = a.split;
     const int ti = a.list[blockIdx.x / G];
     const TrajParams* __restrict__ P = a.tp + ti;
+    if (a.seg && a.seg[ti].done) {
+        if (threadIdx.x == 0 && blockIdx.x % G == 0) a.done_out[blockIdx.x / G] = 1;
+        return;
+    }
     init_counters(sm);
+    if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
     Chain<1024, MIXED, 0, true> c(sm, *P, a);
     c.set_part(blockIdx.x % G, G);
     c.run(ti);

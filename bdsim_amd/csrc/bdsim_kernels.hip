@@ -965,7 +965,6 @@ int bdsim_batch_run(bdsim_batch* b, const bdsim_sim* sim) {
 int bdsim_batch_run_steps(bdsim_batch* b, const bdsim_sim* sim, int max_steps, int resume) {
     if (!b || !sim) return fail(BDSIM_EINVAL, "null argument");
     if (max_steps <= 0) return fail(BDSIM_EINVAL, "max_steps must be positive");
-    if (b->split > 1) return fail(BDSIM_EINVAL, "resumable runs of split chains are not supported");
     if (int rc = check_clock(sim->time_start, sim->time_end, sim->dt)) return rc;
     if (resume) {
         if (!b->seg_open) return fail(BDSIM_EINVAL, "no segment to resume");
@@ -1015,6 +1014,10 @@ int bdsim_batch_run_steps(bdsim_batch* b, const bdsim_sim* sim, int max_steps, i
     a.seg = b->seg;
     a.done_out = b->done;
     a.max_steps = max_steps;
+    if (b->split > 1)
+        return is_mixed(b->precision)
+                   ? launch_split<true>(n, b->split, b->stream, a, b->bar, b->xbuf)
+                   : launch_split<false>(n, b->split, b->stream, a, b->bar, b->xbuf);
     if (is_mixed(b->precision))
         launch_integrate<true>(b->block, dim3(n), b->stream, a, b->dense_nb, -1,
                                b->precision == BDSIM_PRECISION_MIXED_F32_FACTOR, b->N);

@@ -17,8 +17,11 @@
 // ranges fit.
 //
 // Stepping: trajectories are grouped by block size (64..1024 threads, from
-// N); Cholesky chains whose factor fits in LDS (3N <= 96) form two groups of
-// their own on the dense kernel. Each group runs its own sequence of launches on its own stream with a
+// N); Cholesky chains with 3N <= 96 form two groups of their own on the
+// register-factor kernel. Small jobs of long chains (N > 256, few chains)
+// hold G workspace copies per chain and form split groups (one per G): a
+// launch of such a group runs its chains split over G co-resident workgroups
+// each (k_integrate_split) while the device has idle SIMDs, else whole. Each group runs its own sequence of launches on its own stream with a
 // per-launch step budget; a trajectory's segment is resumed launch after
 // launch (SegState), which is bit for bit one uninterrupted integrate call, so
 // a job's result does not depend on what else is resident. poll() retires the
@@ -87,7 +90,8 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 struct SrvJob {
     int64_t id = 0;
     int n_traj = 0, N = 0, block = 64;
-    int nb = 0;                       // > 0: Cholesky chains with the factor in LDS
+    int nb = 0;                       // > 0: Cholesky chains on the register-factor kernel
+    int split = 1;                    // > 1: G workspace copies per chain (a split group)
     // packed per-trajectory parameters (blob/ext offsets relative to the job)
     std::vector<TrajParams> tp;
     std::vector<double> blob;
@@ -122,11 +126,44 @@ struct SrvGroup {
     int64_t cap = 0;
     std::vector<int32_t> h_done;
     int steps = 1;
+    // split groups: workgroups per chain, and the barrier counters and
+    // exchange buffers of a split launch (scap chains)
+    int split = 1;
+    uint32_t* d_bar = nullptr;
+    double* d_xbuf = nullptr;
+    int64_t scap = 0;
 };
 
-// groups 0-4: k_integrate by block size; 5 and 6: the Cholesky chains whose
-// factor fits in LDS (3N <= 64, 3N <= 96)
-constexpr int SRV_GROUPS = 7;
+// groups 0-4: k_integrate by block size; 5 and 6: the Cholesky chains with
+// 3N <= 64, 3N <= 96; 7-11: split groups of chains of 257-320, 321-384,
+// 385-448, 449-512 and 513-1024 beads (G = 5, 6, 7, 8, 8 workgroups per chain:
+// the tiles t = p (mod G) for part p, so G <= T tiles keeps every part busy)
+constexpr int SRV_GROUPS = 12;
+constexpr int SRV_SPLIT0 = 7;
+
+// Workgroups per chain of a job of n_traj chains of N beads over `steps`
+// steps: a split group's G for small, long jobs of long chains (N > 256, at
+// most 16 chains, at least 4 compute units per chain, 2000 steps or more: the
+// jobs whose latency a split shortens), else 1. A split launch sums the
+// partner terms in another order, so such a job's result can differ from an
+// unsplit run in the last bits (within the forms' tolerances).
+// BDSIM_SERVER_SPLIT=0 in the environment turns it off.
+constexpr int SRV_SPLIT_MIN_STEPS = 2000;
+int srv_split_of(int N, int n_traj, int64_t steps) {
+    static const bool enabled = [] {
+        const char* e = std::getenv("BDSIM_SERVER_SPLIT");
+        return !(e && e[0] == '0');
+    }();
+    if (!enabled || !g.coop || N <= 256 || n_traj > 16 || (int64_t)n_traj * 4 > g.n_cu ||
+        steps < SRV_SPLIT_MIN_STEPS)
+        return 1;
+    return std::min(8, (N + 63) / 64);
+}
+
+int split_group_of(int N) {
+    return N <= 320 ? SRV_SPLIT0 : N <= 384 ? SRV_SPLIT0 + 1 : N <= 448 ? SRV_SPLIT0 + 2
+                                   : N <= 512 ? SRV_SPLIT0 + 3 : SRV_SPLIT0 + 4;
+}
 
 int group_of(int block, int nb) {
     if (nb == 4) return 5;
@@ -171,6 +208,8 @@ struct bdsim_server {
             if (gr.stream) (void)hipStreamDestroy(gr.stream);
             (void)hipFree(gr.d_list);
             (void)hipFree(gr.d_done);
+            (void)hipFree(gr.d_bar);
+            (void)hipFree(gr.d_xbuf);
         }
         for (void* p : {(void*)tp, (void*)rng, (void*)status, (void*)cnt, (void*)seg, (void*)R,
                         (void*)ws, (void*)blob, (void*)ext})
@@ -180,13 +219,31 @@ struct bdsim_server {
 
 namespace {
 
+int srv_group_of(const SrvJob& j) {
+    return j.split > 1 ? split_group_of(j.N) : group_of(j.block, j.nb);
+}
+
+// Before a split launch: every part's workspace copy takes the warm Lanczos
+// seed from copy 0 (the only workspace content a step takes from the one
+// before; whole launches keep it in copy 0 alone). One workgroup per chain.
+__global__ void k_seed_fanout(const TrajParams* tp, const int32_t* list, double* ws, int G) {
+    const TrajParams& P = tp[list[blockIdx.x]];
+    const WsLayout wl = ws_layout(P.N, P.hstar > 0.0);
+    const double* src = ws + P.ws_off + wl.S + (int64_t)SLOT_SEED * P.N;
+    for (int p = 1; p < G; ++p) {
+        double* dst = ws + P.ws_off + (int64_t)p * wl.total + wl.S + (int64_t)SLOT_SEED * P.N;
+        for (int e = threadIdx.x; e < 3 * P.N; e += blockDim.x) dst[e] = src[e];
+    }
+}
+
 // Try to place a queued job; false when some range does not fit yet.
 bool srv_admit(bdsim_server* s, SrvJob& j) {
     const int64_t nR = (int64_t)j.n_traj * 3 * j.N;
     const int64_t slot = s->a_slot.alloc(j.n_traj);
     if (slot < 0) return false;
     const int64_t roff = s->a_R.alloc(nR);
-    const int64_t wlen = align_up(j.wsper, WS_ALIGN) * j.n_traj;
+    const int64_t wstride = align_up(j.wsper * j.split, WS_ALIGN);
+    const int64_t wlen = wstride * j.n_traj;
     const int64_t woff = roff < 0 ? -1 : s->a_ws.alloc(wlen);
     const int64_t blen = (int64_t)j.blob.size();
     const int64_t boff = woff < 0 ? -1 : s->a_blob.alloc(blen);
@@ -207,7 +264,7 @@ bool srv_admit(bdsim_server* s, SrvJob& j) {
     for (int i = 0; i < j.n_traj; ++i) {
         TrajParams& t = tps[i];
         t.R_off = roff + (int64_t)i * 3 * j.N;
-        t.ws_off = woff + align_up(j.wsper, WS_ALIGN) * i;
+        t.ws_off = woff + wstride * i;       // copy p of a split chain at + p * wsper
         t.phi_off += boff;
         t.flow_off += boff;
         t.ext_off += eoff;
@@ -218,7 +275,7 @@ bool srv_admit(bdsim_server* s, SrvJob& j) {
     std::vector<SegState> seg(j.n_traj, SegState{j.t0, 0, 0});
     std::vector<int32_t> st(j.n_traj, 0);
     std::vector<Counters> cz(j.n_traj, Counters{});
-    SrvGroup& gr = s->grp[group_of(j.block, j.nb)];
+    SrvGroup& gr = s->grp[srv_group_of(j)];
     hipStream_t q = gr.stream;
     bool ok = hipMemcpyAsync(s->tp + slot, tps.data(), sizeof(TrajParams) * j.n_traj, hipMemcpyHostToDevice, q) == hipSuccess &&
               hipMemcpyAsync(s->R + roff, j.R0.data(), sizeof(double) * nR, hipMemcpyHostToDevice, q) == hipSuccess &&
@@ -299,7 +356,7 @@ int srv_service(bdsim_server* s, int gi) {
     // admit queued jobs of this group, in arrival order
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         SrvJob& j = s->jobs[*it];
-        if (group_of(j.block, j.nb) == gi && srv_admit(s, j)) it = s->queue.erase(it);
+        if (srv_group_of(j) == gi && srv_admit(s, j)) it = s->queue.erase(it);
         else ++it;
     }
     if (gr.active.empty()) return 0;
@@ -337,7 +394,29 @@ int srv_service(bdsim_server* s, int gi) {
     // waves of all groups' active trajectories: the groups run side by side
     long long waves = 0;
     for (const SrvGroup& o : s->grp) waves += (long long)o.active.size() * (o.block / 64);
-    if (is_mixed(s->precision))
+    // a split group runs split while the whole device holds no more than one
+    // wave per SIMD (its parts, workgroups of 1024 threads and most of a
+    // CU's LDS each, then find idle compute units)
+    if (gr.split > 1 && waves <= device_simds()) {
+        if (n > gr.scap) {
+            (void)hipFree(gr.d_bar);
+            (void)hipFree(gr.d_xbuf);
+            gr.d_bar = nullptr;
+            gr.d_xbuf = nullptr;
+            const int64_t c = std::max<int64_t>(n, 2 * gr.scap);
+            if (hipMalloc(&gr.d_bar, sizeof(uint32_t) * c) != hipSuccess ||
+                hipMalloc(&gr.d_xbuf, sizeof(double) * c * split_xbuf_doubles(gr.split)) != hipSuccess)
+                return fail(BDSIM_ENOMEM, "serving: split buffers allocation failed");
+            gr.scap = c;
+        }
+        hipLaunchKernelGGL(k_seed_fanout, dim3((unsigned)n), dim3(256), 0, gr.stream, s->tp, gr.d_list, s->ws,
+                           gr.split);
+        HIP_TRY(hipGetLastError());
+        const int rc = is_mixed(s->precision)
+                           ? launch_split<true>((int)n, gr.split, gr.stream, a, gr.d_bar, gr.d_xbuf)
+                           : launch_split<false>((int)n, gr.split, gr.stream, a, gr.d_bar, gr.d_xbuf);
+        if (rc) return rc;
+    } else if (is_mixed(s->precision))
         launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves,
                                s->precision == BDSIM_PRECISION_MIXED_F32_FACTOR);
     else
@@ -387,12 +466,14 @@ int bdsim_server_create(bdsim_server** out, int precision, int64_t hbm_bytes, in
     s->a_ext.reset(ext_cap);
     s->a_ws.reset(ws_cap / WS_ALIGN * WS_ALIGN);
     s->slot_job.assign(s->slot_cap, -1);
-    const int blocks[SRV_GROUPS] = {64, 128, 256, 512, 1024, 64, 64};
-    const int nbs[SRV_GROUPS] = {0, 0, 0, 0, 0, 4, 6};
+    const int blocks[SRV_GROUPS] = {64, 128, 256, 512, 1024, 64, 64, 512, 512, 512, 512, 1024};
+    const int nbs[SRV_GROUPS] = {0, 0, 0, 0, 0, 4, 6, 0, 0, 0, 0, 0};
+    const int splits[SRV_GROUPS] = {1, 1, 1, 1, 1, 1, 1, 5, 6, 7, 8, 8};
     for (int i = 0; i < SRV_GROUPS; ++i) {
         SrvGroup& gr = s->grp[i];
         gr.block = blocks[i];
         gr.nb = nbs[i];
+        gr.split = splits[i];
         // about the same time per launch in every group: the pair work of a
         // chain step grows with BLOCK^2 spread over BLOCK / 64 waves
         gr.steps = std::max(1, steps_per_launch * 64 / blocks[i]);
@@ -418,6 +499,8 @@ int bdsim_server_submit(bdsim_server* s, int64_t job_id, int n_traj, int nbeads,
     j.N = nbeads;
     j.block = block_for(nbeads);
     j.nb = dense_nb_for(nbeads, phys->hstar, phys->hi_method);
+    j.split = srv_split_of(nbeads, n_traj,
+                           (int64_t)std::floor((sim->time_end - sim->time_start) / sim->dt + 0.5) + 1);
     TrajParams proto;
     if (int rc = pack_traj(*phys, *sim, nbeads, j.blob, j.ext, proto)) return rc;
     j.tp.assign(n_traj, proto);
@@ -427,7 +510,7 @@ int bdsim_server_submit(bdsim_server* s, int64_t job_id, int n_traj, int nbeads,
     j.rng0.resize(n_traj);
     for (int i = 0; i < n_traj; ++i) j.rng0[i] = RngState{rng[i].ix, rng[i].iy, rng[i].seed};
     if ((int64_t)n_traj > s->a_slot.cap() || (int64_t)n_traj * 3 * nbeads > s->a_R.cap() ||
-        align_up(j.wsper, WS_ALIGN) * n_traj > s->a_ws.cap() ||
+        align_up(j.wsper * j.split, WS_ALIGN) * n_traj > s->a_ws.cap() ||
         (int64_t)j.blob.size() > s->a_blob.cap() || (int64_t)j.ext.size() > s->a_ext.cap())
         return fail(BDSIM_ENOMEM, "job larger than the whole memory budget");
     s->jobs.emplace(job_id, std::move(j));

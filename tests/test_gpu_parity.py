@@ -446,17 +446,20 @@ def _batch_segment(gpu, R0, p, prec, sim, cuts):
     return R, [(s.ix, s.iy, s.seed) for s in st], status, [bytes(c) for c in cnt]
 
 
-@pytest.mark.parametrize("kind", ["cheb64", "cheb130", "cholesky30"])
+@pytest.mark.parametrize("kind", ["cheb64", "cheb130", "cholesky30", "split600"])
 def test_resident_segment_cut_into_runs(gpu, kind):
     """bdsim_batch_run_steps: a 30-step segment cut into runs of 7, 13, 9 and
     5 steps (the last finds 1 step left) equals one bdsim_batch_run over it
     bit for bit: positions, generator states, status and counters, both forms
-    (the resumed runs start from the stored time and warm Lanczos seed)."""
-    N = {"cheb64": 50, "cheb130": 130, "cholesky30": 30}[kind]
+    (the resumed runs start from the stored time and warm Lanczos seed; split600:
+    3 chains of N=600 each split over several workgroups, every part resuming
+    from the same stored state)."""
+    N = {"cheb64": 50, "cheb130": 130, "cholesky30": 30, "split600": 600}[kind]
     d = C.phys_dict(spring=2, sqrtb=10.0, hstar=0.25, ev=3 if N == 50 else 1, zstar=0.5, dstar=1.0,
                     method=1 if kind == "cholesky30" else 0, kappa=C.wi_kappa(3.0))
     p = H.Packed(d, N)
-    R0 = np.ascontiguousarray(np.array([C.random_walk(N, 900 + i, 0.9) for i in range(5)]))
+    n = 3 if kind == "split600" else 5
+    R0 = np.ascontiguousarray(np.array([C.random_walk(N, 900 + i, 0.9) for i in range(n)]))
     sim = A.CSim(0.0, 0.029, 1e-3, 1e-3, 1)
     precs = [A.PRECISION_FP64, A.PRECISION_MIXED]
     if kind == "cholesky30":

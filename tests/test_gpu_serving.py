@@ -136,3 +136,23 @@ def test_served_fp64_jobs_match_oracle(gpu, orc):
             err = np.max(np.abs(res.R[k] - want)) / np.max(np.abs(want))
             assert err <= 1e-10, (job.job_id, k, err)
             assert res.rng_states[k] == (st.ix, st.iy, st.seed)
+
+
+@pytest.mark.parametrize("precision", [A.PRECISION_FP64, A.PRECISION_MIXED])
+def test_long_job_of_long_chains_served_split(gpu, precision):
+    """A long job of few long chains (2 x N=300, 2000 steps) is held with a
+    workspace copy per part and, on an otherwise idle device, launched split
+    over 5 workgroups per chain (k_integrate_split, resumed launch after
+    launch, the warm seed handed to every part's copy before each). The split
+    sums partner terms in another order, so the result equals the unsplit
+    integrate call within rounding: 1e-9 relative (FP64), 1e-6 (mixed), the
+    same generator states."""
+    job = make_job(60, 300, 2, 2000, 0.25, 2.0)
+    R, rng, st = reference(gpu, job, precision)
+    with S.Server(precision, hbm_bytes=1 << 30, steps_per_launch=256) as srv:
+        out = S.serve(srv, [job])
+    res = out[job.job_id]
+    assert np.all(res.status == 0) and np.all(st == 0)
+    assert res.rng_states == rng
+    err = np.max(np.abs(res.R - R)) / np.max(np.abs(R))
+    assert err < (1e-9 if precision == A.PRECISION_FP64 else 1e-6), err
