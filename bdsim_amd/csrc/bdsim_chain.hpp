@@ -606,56 +606,104 @@ struct RegLds {
     double row[NB][16];         // row j of each block of block row K, unscaled
     double xs[16 * NB];         // the operand, zero beyond 3N
     double yp[4][16 * NB];      // L x: partial sums of the four lane row groups
+    double pw[2 * 6 * 32];      // RPY weights of the current block row's bead pairs
 };
 
 // The tensor's entries (as dense_factor_ool forms them, one reciprocal square
 // root per entry) from the FP64 positions in LDS, and the lane's place.
+// TAB: the weights come from a per-block-row table of bead pairs (one
+// reciprocal square root per pair; the mixed form); otherwise each entry forms
+// its pair's weights itself (the FP64 form, whose other live state leaves no
+// registers for the table's schedule). Both give the same entries bit for bit.
+template <bool TAB>
 struct RegBuild {
-    lds_cptr<double> R0, R1, R2;
-    int n, c, g;
+    lds_cptr<double> Rb;        // sm.R: component d of bead mu at 64 d + mu
+    lds_ptr<double> pw;         // RegLds::pw
+    int n, N, c, g;
     double hstar, ihstar;
-    // block (J, I) in the accumulator layout
-    __device__ __forceinline__ mfma_d4 operator()(int J, int I) const {
+
+    // the RPY weights of one pair from its squared separation (clamped)
+    __device__ __forceinline__ void weights(double sq, double& om1, double& om2) const {
         const double RPI = 1.7724538509055160273;
         const double TRPI = 2.0 * RPI, TPI = 2.0 * PI, C1 = TPI / 3.0, C3 = 0.75 / RPI * 0.375,
                      C7 = 0.09375 / RPI, RPI3by4 = RPI * 0.75;
+        if (sq < 1.0e-12) sq = 1.0e-12;
+        const double ir = rsqrt(sq);
+        const double ir2 = ir * ir;
+        const double rs = sq * ir;
+        const double hsbyrs = hstar * ir, rsbyhs = rs * ihstar;
+        const double hsbyrs2 = hsbyrs * hsbyrs;
+        if (rsbyhs >= TRPI) {
+            om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
+            om2 = RPI3by4 * (hsbyrs * ir2) * (1.0 - TPI * hsbyrs2);
+        } else {
+            om1 = 1.0 - C3 * rsbyhs;
+            om2 = C7 * rsbyhs * ir2;
+        }
+    }
+
+    // The RPY weights (om1, om2) of the bead pairs block row K needs: row
+    // beads mu in [mlo, mlo + 6) (rows 16 K .. 16 K + 15) against beads nu in
+    // [mlo, N), one reciprocal square root per pair (hydrodynamics.cpp:27-43
+    // as dense_factor_ool forms them), pair (a, b) at 2 (32 a + b).
+    __device__ __forceinline__ void table(int K) const {
+        if constexpr (!TAB) return;
+        const int mlo = 16 * K / 3;
+        const int mhi = min(N - 1, (16 * K + 15) / 3);
+        const int W = N - mlo;
+        const int np = (mhi - mlo + 1) * W;
+        __syncthreads();            // the previous block row's entries are built
+#pragma unroll 1
+        for (int p = threadIdx.x; p < np; p += 64) {
+            const int a = p / W, bb = p - a * W;
+            const int mu = mlo + a, nu = mlo + bb;
+            const double b0 = Rb[mu] - Rb[nu], b1 = Rb[64 + mu] - Rb[64 + nu],
+                         b2 = Rb[128 + mu] - Rb[128 + nu];
+            double om1, om2;
+            weights(b0 * b0 + b1 * b1 + b2 * b2, om1, om2);
+            pw[2 * (32 * a + bb)] = om1;
+            pw[2 * (32 * a + bb) + 1] = om2;
+        }
+        __syncthreads();
+    }
+
+    // block (K, I) of block row K (after table(K)) in the accumulator layout:
+    // entry om2 b_q b_p + [p == q] om1 of the bead pair (mu, nu) = (i / 3,
+    // j / 3), b = R_mu - R_nu; the identity on a bead's own 3 x 3 block and on
+    // the padding
+    __device__ __forceinline__ mfma_d4 operator()(int K, int I) const {
         // the lane's place as an opaque value: otherwise the index arithmetic
         // of all 21 blocks is hoisted out of the step loop and held (spilled)
         // across the whole step
         int cl = c, gl = g;
         asm volatile("" : "+v"(cl), "+v"(gl));
+        const int mlo = 16 * K / 3;
         const int j = 16 * I + cl;
         const int jb = j / 3;
         const int qc = j - 3 * jb;
-        const int nu = j < n ? jb : 0;
-        const double rn0 = R0[nu], rn1 = R1[nu], rn2 = R2[nu];
+        const int nu = j < n ? jb : mlo;
         mfma_d4 v;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 16 * J + gl + 4 * q;
+            const int i = 16 * K + gl + 4 * q;
             const int ib = i / 3;
             const int p = i - 3 * ib;
-            const int mu = i < n ? ib : 0;
-            const double b0 = R0[mu] - rn0, b1 = R1[mu] - rn1, b2 = R2[mu] - rn2;
-            double sq = b0 * b0 + b1 * b1 + b2 * b2;
-            if (sq < 1.0e-12) sq = 1.0e-12;
-            const double ir = rsqrt(sq);
-            const double ir2 = ir * ir;
-            const double rs = sq * ir;
-            const double hsbyrs = hstar * ir, rsbyhs = rs * ihstar;
-            const double hsbyrs2 = hsbyrs * hsbyrs;
-            double om1, om2;
-            if (rsbyhs >= TRPI) {
-                om1 = RPI3by4 * hsbyrs * (1.0 + C1 * hsbyrs2);
-                om2 = RPI3by4 * (hsbyrs * ir2) * (1.0 - TPI * hsbyrs2);
+            const int mu = i < n ? ib : mlo;
+            double om1, om2, bp, bq;
+            if constexpr (TAB) {
+                const int w = 2 * (32 * (mu - mlo) + (nu - mlo));
+                om1 = pw[w];
+                om2 = pw[w + 1];
+                bp = Rb[64 * p + mu] - Rb[64 * p + nu];
+                bq = Rb[64 * qc + mu] - Rb[64 * qc + nu];
             } else {
-                om1 = 1.0 - C3 * rsbyhs;
-                om2 = C7 * rsbyhs * ir2;
+                const double b0 = Rb[mu] - Rb[nu], b1 = Rb[64 + mu] - Rb[64 + nu],
+                             b2 = Rb[128 + mu] - Rb[128 + nu];
+                weights(b0 * b0 + b1 * b1 + b2 * b2, om1, om2);
+                bp = p == 0 ? b0 : (p == 1 ? b1 : b2);
+                bq = qc == 0 ? b0 : (qc == 1 ? b1 : b2);
             }
-            const double bp = p == 0 ? b0 : (p == 1 ? b1 : b2);
-            const double bq = qc == 0 ? b0 : (qc == 1 ? b1 : b2);
             double e = om2 * bq * bp + (p == qc ? om1 : 0.0);
-            // one bead's own 3 x 3 block and the padding: the identity
             if (i >= n || j >= n || ib == jb) e = i == j ? 1.0 : 0.0;
             // formed here: sunk to its first use, the entry would keep its
             // separation and weights live across the whole block row
@@ -672,13 +720,14 @@ struct RegBuild {
 // K keeps every block index a compile-time constant, so the blocks stay in
 // registers; U_JK dies once row K is formed, so (K + 1)(NB - K) blocks are
 // live at most (12 at NB = 6). false on a non-positive pivot.
-template <int K, int NB>
-__device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<double> yp, const RegBuild& build,
+template <int K, int NB, bool TAB>
+__device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<double> yp, const RegBuild<TAB>& build,
                                                lds_ptr<double> rowb, lds_cptr<double> xs, Counters& cnt) {
     if constexpr (K == NB) {
         return true;
     } else {
         const int n = build.n, c = build.c, g = build.g;
+        build.table(K);
 #pragma unroll
         for (int I = K; I < NB; ++I) M[K][I] = build(K, I);
 #pragma unroll
@@ -768,7 +817,7 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<dou
         }
         // block rows of padding stay unit rows
         if (16 * (K + 1) >= n) return true;
-        return regf_block_row<K + 1, NB>(M, yp, build, rowb, xs, cnt);
+        return regf_block_row<K + 1, NB, TAB>(M, yp, build, rowb, xs, cnt);
     }
 }
 
@@ -784,7 +833,8 @@ __device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB
     lds_ptr<double> rowb = as_lds_mut(&rlp->row[0][0]);
     lds_ptr<double> xs = as_lds_mut(rlp->xs);
     lds_ptr<double> yp = as_lds_mut(&rlp->yp[0][0]);
-    const RegBuild build{as_lds(sm.R[0]), as_lds(sm.R[1]), as_lds(sm.R[2]), n, c, g, hstar, 1.0 / hstar};
+    static_assert(BLOCK == 64, "sm.R rows of 64 beads");
+    const RegBuild<MIXED> build{as_lds(sm.R[0]), as_lds_mut(rlp->pw), n, N, c, g, hstar, 1.0 / hstar};
 
     __syncthreads();
     if (tid < N) {
@@ -796,7 +846,7 @@ __device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB
     __syncthreads();
 
     mfma_d4 M[NB][NB];
-    if (!regf_block_row<0, NB>(M, yp, build, rowb, xs, sm.cnt)) return false;
+    if (!regf_block_row<0, NB, MIXED>(M, yp, build, rowb, xs, sm.cnt)) return false;
     __syncthreads();
     y[0] = y[1] = y[2] = 0.0;
     if (tid < N)
