@@ -601,12 +601,16 @@ __device__ __noinline__ D3 lower_matvec_lds(lds_ptr<T> Lb, lds_ptr<double> xs, i
 // after its last use: at most 12 of the 21 blocks are live at NB = 6 (96
 // VGPRs), which with the rest of the step fits the register budget of two
 // waves per SIMD, and the chain needs 13 KB of LDS instead of 53 KB.
-template <int NB>
+// TAB: with the pair-weight table of RegBuild<true>. slots: the step's state
+// slots (F^S, F^E, dW; 9 N doubles, N <= 32) as the chain's workspace in LDS,
+// so that the step's slot round trips stay on chip.
+template <int NB, bool TAB>
 struct RegLds {
     double row[NB][16];         // row j of each block of block row K, unscaled
     double xs[16 * NB];         // the operand, zero beyond 3N
     double yp[4][16 * NB];      // L x: partial sums of the four lane row groups
-    double pw[2 * 6 * 32];      // RPY weights of the current block row's bead pairs
+    double pw[TAB ? 2 * 6 * 32 : 1];   // RPY weights of the current block row's bead pairs
+    double slots[9 * 32];
 };
 
 // The tensor's entries (as dense_factor_ool forms them, one reciprocal square
@@ -825,7 +829,7 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<dou
 // false on a non-positive pivot, taken in column order, its index and value
 // in the counters.
 template <int BLOCK, bool MIXED, int NB>
-__device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB>* rlp, int N,
+__device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB, MIXED>* rlp, int N,
                                                double hstar, const double x[3], double y[3]) {
     static_assert(BLOCK == 64, "one wave per chain");
     const int n = 3 * N, tid = threadIdx.x;
@@ -917,7 +921,7 @@ struct Chain {
     uint32_t epoch = 0;         // SPLIT: exchanges done
     bool sync_fail = false;     // SPLIT: a barrier timed out
     bool dense_ws = false;      // this step's factor went to the workspace
-    using RL = RegLds<(NB > 0 ? NB : 1)>;
+    using RL = RegLds<(NB > 0 ? NB : 1), MIXED>;
     RL* rl = nullptr;           // REGF: the register factor's LDS buffers
 
     __device__ __forceinline__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
@@ -2232,12 +2236,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
 }
 
 // Chains of the Cholesky method with 3N <= 16 NB, the FP64 factor in
-// registers (dense_bdw_regs), 13 KB of LDS per chain. WPE: the waves per SIMD
-// the register allocation is held to.
+// registers (dense_bdw_regs), 16-20 KB of LDS per chain (8 chains fit a CU).
+// WPE: the waves per SIMD the register allocation is held to.
 template <bool MIXED, int NB, int WPE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_integrate_dreg(KArgs a) {
     __shared__ Smem<64, MIXED> sm;
-    __shared__ RegLds<NB> rl;
+    __shared__ RegLds<NB, MIXED> rl;
     const int ti = a.list[blockIdx.x];
     const TrajParams* __restrict__ P = a.tp + ti;
     if (a.seg && a.seg[ti].done) {
@@ -2246,7 +2250,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
     }
     init_counters(sm);
     if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
-    Chain<64, MIXED, NB, false, double, false, true> c(sm, *P, a, nullptr, nullptr, nullptr, &rl);
+    // the workspace in LDS is the state slots alone: the Cholesky step uses
+    // no other part of it (no Lanczos basis, no Chebyshev coefficients)
+    Chain<64, MIXED, NB, false, double, true, true> c(sm, *P, a, nullptr, nullptr, rl.slots, &rl);
     c.run(ti);
 }
 
