@@ -1105,6 +1105,10 @@ struct Chain {
             apply_pairs(v, y);
             return;
         }
+        if constexpr (!MIXED && BLOCK == 64 && !SPLIT) {
+            apply_pairs64(v, y);
+            return;
+        }
         if constexpr (S::TILES) {
             apply_tiles(v, y);
             if constexpr (SPLIT) {
@@ -1238,6 +1242,65 @@ struct Chain {
             flush(0, fx, fy, fz);
         }
         if (own) { y[0] = v[0] + ax; y[1] = v[1] + ay; y[2] = v[2] + az; }
+        else { y[0] = y[1] = y[2] = 0.0; }
+    }
+
+    // D x for a single-wave chain in the FP64 form, each unordered bead pair
+    // evaluated once: the walk of apply_pairs (lane mu with partner
+    // (mu + k) mod N for k = K..1, and for even N k = N/2 from both ends with
+    // own-row terms only; the partner row's term on an accumulator that moves
+    // one lane up per step), in FP64 throughout and with the partner read at
+    // a wrapped index of the staged positions and operand. After step k = 1
+    // lane j's moving accumulator holds row j's terms. y = x + the sum
+    // (identity diagonal blocks, as Diffusion::apply has them).
+    __device__ __forceinline__ void apply_pairs64(const double v[3], double y[3]) {
+        RpyW<double> W;
+        W.init(P.rpy_d);
+        const lds_cptr<double> r0 = as_lds(sm.R[0]), r1 = as_lds(sm.R[1]), r2 = as_lds(sm.R[2]);
+        const lds_cptr<double> v0 = as_lds(sm.X[0]), v1 = as_lds(sm.X[1]), v2 = as_lds(sm.X[2]);
+        const int me = own ? tid : 0;
+        const double rx = r0[me], ry = r1[me], rz = r2[me];
+        const double xi = v0[me], yi = v1[me], zi = v2[me];
+        // lane j takes the moving accumulator from lane j - 1 (mod N)
+        const int rot = own ? 4 * (tid == 0 ? N - 1 : tid - 1) : 4 * tid;
+        auto perm = [](int addr, double x) {
+            const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(x));
+            const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(x));
+            return __hiloint2double(hi, lo);
+        };
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        auto pair = [&](int nu, bool both) {
+            const double bx = r0[nu] - rx, by = r1[nu] - ry, bz = r2[nu] - rz;
+            double om1, om2;
+            W.weights(bx * bx + by * by + bz * bz, om1, om2);
+            const double xx = v0[nu], xy = v1[nu], xz = v2[nu];
+            const double t = om2 * (bx * xx + by * xy + bz * xz);
+            ax = fma(om1, xx, ax); ax = fma(t, bx, ax);
+            ay = fma(om1, xy, ay); ay = fma(t, by, ay);
+            az = fma(om1, xz, az); az = fma(t, bz, az);
+            if (both) {
+                const double u = om2 * (bx * xi + by * yi + bz * zi);
+                sx = fma(om1, xi, sx); sx = fma(u, bx, sx);
+                sy = fma(om1, yi, sy); sy = fma(u, by, sy);
+                sz = fma(om1, zi, sz); sz = fma(u, bz, sz);
+                sx = perm(rot, sx); sy = perm(rot, sy); sz = perm(rot, sz);
+            }
+        };
+        if ((N & 1) == 0 && N >= 2) {
+            int nu = me + N / 2;
+            if (nu >= N) nu -= N;
+            pair(nu, false);
+        }
+        const int K = (N - 1) / 2;
+        int nu = me + K;
+        if (nu >= N) nu -= N;
+#pragma unroll 4
+        for (int k = K; k >= 1; --k) {
+            pair(nu, true);
+            nu = (nu == 0) ? N - 1 : nu - 1;
+        }
+        if (own) { y[0] = v[0] + (ax + sx); y[1] = v[1] + (ay + sy); y[2] = v[2] + (az + sz); }
         else { y[0] = y[1] = y[2] = 0.0; }
     }
 
