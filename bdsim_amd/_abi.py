@@ -280,6 +280,13 @@ def load_gpu():
     """The initialised GPU library. Raises GPUUnavailable -- never falls back."""
     global _gpu
     if _gpu is None:
+        # The batch path (one stream per block size) and the serving loop (one
+        # per group, up to 12) launch side by side on streams of their own;
+        # HIP runs the launches of streams that share one of its
+        # GPU_MAX_HW_QUEUES hardware queues (4 by default) one after another.
+        # Unless the process has chosen a value, ask for 16 (read by HIP at its
+        # first call in the process, so it has no effect once HIP is running).
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         lib = load_gpu_library()
         rc = lib.bdsim_init(int(os.environ.get("BDSIM_DEVICE", "0")))
         if rc != 0:

@@ -55,6 +55,15 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
+# HIP places each stream on one of GPU_MAX_HW_QUEUES hardware queues (4 by
+# default) and runs the launches of streams that share a queue one after
+# another. The serving loop runs up to 12 groups (one per block size and
+# method) side by side, each on a stream of its own, so this process asks for
+# 16 queues; it must be set before the first HIP call of the process. Launches
+# on a single stream (the throughput legs) do not depend on it.
+HW_QUEUES = 16
+os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 from bdsim_amd import _abi as A  # noqa: E402
 from bdsim_amd.workloads import (N_BEADS, N_TRAJ, WI, headline_phys, kappas, shear_rate,  # noqa: E402
                                  cholesky_workload, long_workload, start_configs,
@@ -283,7 +292,7 @@ def serving_leg(n_jobs, step_scale):
     bad = sum(int(np.sum(r.status != 0)) for r in rep["results"].values())
     return {"p50_s": rep["p50_s"], "p99_s": rep["p99_s"], "makespan_s": rep["makespan_s"],
             "bead_steps": rep["bead_steps"], "bead_steps_per_s": rep["bead_steps_per_s"],
-            "failed_trajectories": bad,
+            "failed_trajectories": bad, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "sample": (f"first {n_jobs} jobs of the serving stream (Poisson 50 jobs/s, seed 3; "
                        f"N log-uniform [10,500], 1..64 trajectories, Wi log-uniform [0.1,100], "
                        f"h* uniform [0,0.3], FENE b=100, Gauss EV, dt=1e-3) with n_steps "
