@@ -20,7 +20,10 @@ under "w<block>" the width multiplier) applies with `apply` (bdsim_tune_set,
 bdsim_tune_split, bdsim_tune_width) and round-trips through JSON (`save` /
 `load`). Register budgets change time only (the same operations, another
 register allocation); G and the width also change the order in which partner
-sums are added (the width: exactly, through fixed-point sums).
+sums are added (the width: exactly, through fixed-point sums). Without a
+budget of its own the 64-thread bucket's chains of up to 51 beads (mixed
+form) run with the step's pair weights in an LDS table (k_integrate_ptab);
+an explicit budget selects the kernel that forms them in each mat-vec.
 """
 import json
 import time

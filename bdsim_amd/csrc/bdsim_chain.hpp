@@ -895,8 +895,11 @@ __device__ __noinline__ bool chain_barrier(uint32_t* bar, uint32_t target) {
 // REGF (NB > 0, FT double): the Cholesky method's factor in registers
 // (dense_bdw_regs) instead of LDS; the step draws the noise first and factors
 // after, restoring the generator state when the factor fails.
+// PTAB (mixed form, 64-thread chains): the RPY weights of the step's bead
+// pairs in an LDS table (pair_tab), formed once per step in stage_R and read
+// by every mat-vec of the step instead of recomputed.
 template <int BLOCK, bool MIXED, int NB = 0, bool SPLIT = false, typename FT = double,
-          bool LWS = false, bool REGF = false>
+          bool LWS = false, bool REGF = false, bool PTAB = false>
 struct Chain {
     static constexpr int NW = BLOCK / 64;
 #ifndef BDSIM_MIXED_RITZ_F32
@@ -923,10 +926,12 @@ struct Chain {
     bool dense_ws = false;      // this step's factor went to the workspace
     using RL = RegLds<(NB > 0 ? NB : 1), MIXED>;
     RL* rl = nullptr;           // REGF: the register factor's LDS buffers
+    static_assert(!PTAB || S::PAIRS, "the pair-weight table serves the single-wave pair walk");
+    lds_ptr<f32x2> ptab;        // PTAB: (om1, om2) of pair (mu, mu + k) at mu * pt_stride() + k - 1
 
     __device__ __forceinline__ Chain(Smem<BLOCK, MIXED>& s, const TrajParams& p, const KArgs& a,
                      FT* dense_blk = nullptr, double* dense_xs = nullptr, double* lds_ws = nullptr,
-                     void* reg_lds = nullptr)
+                     void* reg_lds = nullptr, void* pair_tab = nullptr)
         : sm(s), P(p), A(a), N(p.N), tid(threadIdx.x), n3(3 * p.N), own((int)threadIdx.x < p.N),
           hi(p.hstar > 0.0), wl(ws_layout(p.N, p.hstar > 0.0)) {
         wsg = as_gbl(A.ws + P.ws_off);
@@ -935,7 +940,13 @@ struct Chain {
         dlb = as_lds_mut(dense_blk);
         rl = static_cast<RL*>(reg_lds);
         dxs = as_lds_mut(dense_xs);
+        ptab = as_lds_mut(static_cast<f32x2*>(pair_tab));
     }
+
+    // PTAB: a lane's row of the table holds floor(N / 2) entries, the walk's
+    // steps k = 1..N/2; the row stride is odd, so that the 64 lanes' reads of
+    // one step spread over the banks.
+    __device__ __forceinline__ int pt_stride() const { return (N >> 1) | 1; }
 
     __device__ __forceinline__ double bsum(double v) { return block_sum<NW>(v, sm.Red, par); }
     __device__ __forceinline__ void bump(int64_t& c, int64_t v = 1) { if (tid == 0) c += v; }
@@ -1004,6 +1015,26 @@ struct Chain {
                 sm.Pa[tid + N].x = rx; sm.Pa[tid + N].y = ry; sm.Pa[tid + N].z = rz;
             }
             __syncthreads();
+            if constexpr (PTAB) {
+                // the weights apply_pairs forms for the pair (mu, mu + k),
+                // the same operations on the same FP32 separations
+                if (hi && own) {
+                    RpyW<float> W;
+                    W.init(P.rpy_f);
+                    const lds_cptr<f32x4> pa = as_lds(&sm.Pa[0]) + tid;
+                    const f32x4 mine = pa[0];
+                    const lds_ptr<f32x2> row = ptab + tid * pt_stride();
+                    const int S2 = N >> 1;
+                    for (int k = 1; k <= S2; ++k) {
+                        const f32x4 q = pa[k];
+                        const float bx = q.x - mine.x, by = q.y - mine.y, bz = q.z - mine.z;
+                        float om1, om2;
+                        W.weights_floor(fmaf(bz, bz, fmaf(by, by, fmaf(bx, bx, W.clamp))), om1, om2);
+                        row[k - 1] = f32x2{om1, om2};
+                    }
+                }
+                // apply() synchronises before the walk reads the table
+            }
         }
     }
 
@@ -1178,6 +1209,8 @@ struct Chain {
         const int me = own ? tid : 0;
         const lds_cptr<f32x4> pa = as_lds(&sm.Pa[0]) + me;
         const lds_cptr<f32x2> pb = as_lds(&sm.Pb[0]) + me;
+        // PTAB: this lane's row of the step's weights, entry k - 1 for step k
+        const lds_cptr<f32x2> pt = PTAB ? (lds_cptr<f32x2>)ptab + me * pt_stride() - 1 : nullptr;
         const f32x4 mine = pa[0];
         const f32x2 mine2 = pb[0];
         const float rx = mine.x, ry = mine.y, rz = mine.z;
@@ -1199,7 +1232,13 @@ struct Chain {
             // reference's clamp max(|b|^2, 1e-12) (the same value to FP32
             // resolution wherever |b|^2 > 1e-5)
             float om1, om2;
-            W.weights_floor(fmaf(bz, bz, fmaf(by, by, fmaf(bx, bx, W.clamp))), om1, om2);
+            if constexpr (PTAB) {
+                const f32x2 w = pt[k];
+                om1 = w.x;
+                om2 = w.y;
+            } else {
+                W.weights_floor(fmaf(bz, bz, fmaf(by, by, fmaf(bx, bx, W.clamp))), om1, om2);
+            }
             const float xx = q.w, xy = q2.x, xz = q2.y;
             const float t = om2 * (bx * xx + by * xy + bz * xz);
             fx = fmaf(om1, xx, fx); fx = fmaf(t, bx, fx);
@@ -2255,6 +2294,26 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)
     init_counters(sm);
     if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];   // counts run on across launches
     Chain<BLOCK, MIXED> c(sm, *P, a);
+    c.run(ti);
+}
+
+// 64-thread chains of the mixed form with the step's RPY pair weights in an
+// LDS table (dynamic LDS: N * ((N / 2) | 1) float pairs, N the largest chain
+// of the launch): the walk of each mat-vec reads them instead of forming them.
+template <int WPE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_integrate_ptab(KArgs a) {
+    __shared__ Smem<64, true> sm;
+    extern __shared__ f32x2 ptab_lds[];
+    const int ti = a.list[blockIdx.x];
+    const TrajParams* __restrict__ P = a.tp + ti;
+    if (a.seg && a.seg[ti].done) {
+        if (threadIdx.x == 0) a.done_out[blockIdx.x] = 1;
+        return;
+    }
+    init_counters(sm);
+    if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
+    Chain<64, true, 0, false, double, false, false, true> c(sm, *P, a, nullptr, nullptr, nullptr, nullptr,
+                                                          ptab_lds);
     c.run(ti);
 }
 

@@ -330,6 +330,24 @@ size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
     return (size_t)n_wg <= (size_t)g.n_cu * per_cu ? dyn : 0;
 }
 
+// Dynamic LDS for the pair-weight table of k_integrate_ptab (mixed form,
+// 64-thread chains, largest N = nmax): N * ((N / 2) | 1) float pairs, used
+// when the table and the chain's static LDS together leave room for 8 chains
+// per compute unit (N <= 51), which the two-waves-per-SIMD register budget
+// then holds; 0: the walk forms the weights itself. BDSIM_PAIR_TABLE=0 in the
+// environment turns the table off, =2 uses it at every N <= 64.
+constexpr size_t PTAB_LDS_MAX = LDS_PER_CU / 8;
+size_t ptab_bytes_for(int nmax) {
+    static const int mode = [] {
+        const char* e = std::getenv("BDSIM_PAIR_TABLE");
+        return e ? std::atoi(e) : 1;
+    }();
+    if (mode == 0 || nmax < 2 || nmax > 64) return 0;
+    const size_t bytes = sizeof(f32x2) * (size_t)nmax * (size_t)((nmax >> 1) | 1);
+    if (mode != 2 && sizeof(Smem<64, true>) + bytes > PTAB_LDS_MAX) return 0;
+    return bytes;
+}
+
 // Workgroup width per block size set by the caller (bdsim_tune_width): the
 // chains of that block run on workgroups of block x m threads (at most 1024),
 // 0 = the library's choice. Index as g_tune.
@@ -379,7 +397,11 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     const int tuned = ti >= 0 ? g_tune[ti].load() : 0;
     if (block == 64) {
         const size_t lws = tuned ? 0 : lws_bytes_for<MIXED>(nmax, grid.x, waves);
-        launch_integrate_small<MIXED>(tuned ? tuned : wpe_choice(waves), 0, false, grid, stream, a, lws);
+        const size_t pt = (MIXED && !lws && !tuned) ? ptab_bytes_for(nmax) : 0;
+        const int wpe = tuned ? tuned : wpe_choice(waves);
+        // with the table at most 8 chains share a compute unit: the register
+        // budget of 2 waves per SIMD (1 for a launch of at most one per SIMD)
+        launch_integrate_small<MIXED>(pt ? std::min(wpe, 2) : wpe, 0, false, grid, stream, a, lws, pt);
         return;
     }
     // Multi-wave chains: the register budget of 4 waves per SIMD when the

@@ -14,10 +14,12 @@ namespace bdgpu {
 // kernel with its factor in LDS, k_integrate_dense<64, MIXED, nb, FT>, FT
 // float for the mixed form's FP32-factor variant (f32_factor), else double.
 // lws_bytes > 0: k_integrate_lws<MIXED> with that much dynamic LDS for the
-// hot part of the workspace (few chains; nb = 0).
+// hot part of the workspace (few chains; nb = 0). ptab_bytes > 0 (mixed form,
+// nb = 0, no lws): k_integrate_ptab<wpe> with that much dynamic LDS for the
+// step's pair-weight table.
 template <bool MIXED>
 void launch_integrate_small(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
-                            const KArgs& a, size_t lws_bytes = 0);
+                            const KArgs& a, size_t lws_bytes = 0, size_t ptab_bytes = 0);
 
 // Cholesky chains with the FP64 factor in registers: k_integrate_dreg<MIXED,
 // nb>, nb = 4 or 6 (k_integrate_dreg.hip).

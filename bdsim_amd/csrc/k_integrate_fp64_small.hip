@@ -7,7 +7,7 @@ namespace bdgpu {
 
 template <>
 void launch_integrate_small<false>(int wpe, int nb, bool, dim3 grid, hipStream_t stream,
-                                    const KArgs& a, size_t lws_bytes) {
+                                    const KArgs& a, size_t lws_bytes, size_t) {
     if (lws_bytes > 0 && nb == 0) {
         hipLaunchKernelGGL((k_integrate_lws<false>), grid, dim3(64), lws_bytes, stream, a);
         return;

@@ -7,9 +7,14 @@ namespace bdgpu {
 
 template <>
 void launch_integrate_small<true>(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
-                                   const KArgs& a, size_t lws_bytes) {
+                                   const KArgs& a, size_t lws_bytes, size_t ptab_bytes) {
     if (lws_bytes > 0 && nb == 0) {
         hipLaunchKernelGGL((k_integrate_lws<true>), grid, dim3(64), lws_bytes, stream, a);
+        return;
+    }
+    if (ptab_bytes > 0 && nb == 0) {
+        if (wpe == 1) hipLaunchKernelGGL((k_integrate_ptab<1>), grid, dim3(64), ptab_bytes, stream, a);
+        else hipLaunchKernelGGL((k_integrate_ptab<2>), grid, dim3(64), ptab_bytes, stream, a);
         return;
     }
     if ((nb == 4 || nb == 6) && !f32_factor && dense_reg_enabled()) {

@@ -94,3 +94,31 @@ def test_tune_workgroups_per_long_chain(gpu, orc):
         autotune.reset(gpu)
     assert gpu.bdsim_tune_split_get() == 0
     assert gpu.bdsim_tune_split(17) != 0 and gpu.bdsim_tune_split(-1) != 0
+
+
+def test_pair_weight_table_matches_the_walk(gpu):
+    """Mixed-form chains of up to 51 beads run on k_integrate_ptab (the step's
+    RPY weights in an LDS table, formed once per step); an explicit register
+    budget for the 64-thread bucket selects k_integrate, whose walk forms the
+    weights itself. The table holds the same FP32 operations' results, so the
+    two agree bit for bit, positions and generator states, at every N from 2
+    to 51 (odd and even: the half step k = N/2), with and without HI."""
+    rs = np.random.default_rng(11)
+    Ns = np.array([2, 3, 4, 5, 10, 17, 30, 31, 32, 49, 50, 51] * 4, dtype=np.int32)
+    R, phys = [], []
+    for i, n in enumerate(Ns):
+        R.append(W._walk(rs, int(n)))
+        c = W.headline_phys()
+        c.number_of_beads = int(n)
+        c.hstar = 0.0 if i % 7 == 3 else 0.3
+        c.flow.kappa0[1] = W.shear_rate(W.WI[i % len(W.WI)], int(n))
+        phys.append(c)
+    try:
+        autotune.reset(gpu)
+        base, base_rng = _run(gpu, Ns, R, phys, 1e-3, steps=12)
+        for w in autotune.CANDIDATES[64]:
+            autotune.apply({64: w}, gpu)
+            out, rng = _run(gpu, Ns, R, phys, 1e-3, steps=12)
+            assert np.array_equal(out, base) and rng == base_rng, w
+    finally:
+        autotune.reset(gpu)
