@@ -2320,8 +2320,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 // 64-thread chains of a small launch with the hot part of the workspace in
 // dynamic LDS (at least 8 ws_layout(N, true).dense bytes, N the largest
 // chain of the launch; chains of the Cholesky method with the factor in LDS
-// are not launched here).
-template <bool MIXED>
+// are not launched here). PT (mixed form): the step's pair-weight table
+// follows the chain's workspace part (N * ((N / 2) | 1) float pairs more).
+template <bool MIXED, bool PT = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) k_integrate_lws(KArgs a) {
     __shared__ Smem<64, MIXED> sm;
     extern __shared__ double lws[];
@@ -2333,7 +2334,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1))) k_
     }
     init_counters(sm);
     if (a.seg && threadIdx.x == 0) sm.cnt = a.cnt[ti];
-    Chain<64, MIXED, 0, false, double, true> c(sm, *P, a, nullptr, nullptr, lws);
+    Chain<64, MIXED, 0, false, double, true, false, PT> c(sm, *P, a, nullptr, nullptr, lws, nullptr,
+                                                          PT ? lws + ws_layout(P->N, true).dense : nullptr);
     if (a.seg && a.seg[ti].seed_valid) c.ws_to_lds();
     c.run(ti);
     c.ws_to_hbm();

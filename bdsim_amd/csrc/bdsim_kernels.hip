@@ -317,6 +317,7 @@ int dense_nb_for(int N, double hstar, int hi_method) {
 // dynamic size a launch may ask for without a kernel attribute).
 constexpr size_t LWS_MAX_TOTAL = 64 * 1024;
 constexpr size_t LDS_PER_CU = 160 * 1024;
+size_t ptab_bytes_for(int nmax, bool with_lws = false);
 template <bool MIXED>
 size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
     static const bool enabled = [] {
@@ -324,10 +325,12 @@ size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
         return !(e && e[0] == '0');
     }();
     if (!enabled || nmax <= 0 || waves != (long long)n_wg) return 0;
-    const size_t dyn = sizeof(double) * (size_t)ws_layout(nmax, true).dense;
+    // the mixed form's pair-weight table, when used, follows the workspace part
+    const size_t dyn = sizeof(double) * (size_t)ws_layout(nmax, true).dense +
+                       (MIXED ? ptab_bytes_for(nmax, true) : 0);
     if (sizeof(Smem<64, MIXED>) + dyn > LWS_MAX_TOTAL) return 0;
     const size_t per_cu = std::min<size_t>(4, LDS_PER_CU / (sizeof(Smem<64, MIXED>) + dyn));
-    return (size_t)n_wg <= (size_t)g.n_cu * per_cu ? dyn : 0;
+    return (size_t)n_wg <= (size_t)g.n_cu * per_cu ? sizeof(double) * (size_t)ws_layout(nmax, true).dense : 0;
 }
 
 // Dynamic LDS for the pair-weight table of k_integrate_ptab (mixed form,
@@ -336,15 +339,17 @@ size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
 // per compute unit (N <= 51), which the two-waves-per-SIMD register budget
 // then holds; 0: the walk forms the weights itself. BDSIM_PAIR_TABLE=0 in the
 // environment turns the table off, =2 uses it at every N <= 64.
+// with_lws: the table of a small launch that keeps the workspace in LDS too
+// (k_integrate_lws, at most 4 chains per compute unit): used at every N.
 constexpr size_t PTAB_LDS_MAX = LDS_PER_CU / 8;
-size_t ptab_bytes_for(int nmax) {
+size_t ptab_bytes_for(int nmax, bool with_lws) {
     static const int mode = [] {
         const char* e = std::getenv("BDSIM_PAIR_TABLE");
         return e ? std::atoi(e) : 1;
     }();
     if (mode == 0 || nmax < 2 || nmax > 64) return 0;
     const size_t bytes = sizeof(f32x2) * (size_t)nmax * (size_t)((nmax >> 1) | 1);
-    if (mode != 2 && sizeof(Smem<64, true>) + bytes > PTAB_LDS_MAX) return 0;
+    if (!with_lws && mode != 2 && sizeof(Smem<64, true>) + bytes > PTAB_LDS_MAX) return 0;
     return bytes;
 }
 
@@ -397,7 +402,7 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     const int tuned = ti >= 0 ? g_tune[ti].load() : 0;
     if (block == 64) {
         const size_t lws = tuned ? 0 : lws_bytes_for<MIXED>(nmax, grid.x, waves);
-        const size_t pt = (MIXED && !lws && !tuned) ? ptab_bytes_for(nmax) : 0;
+        const size_t pt = (MIXED && !tuned) ? ptab_bytes_for(nmax, lws > 0) : 0;
         const int wpe = tuned ? tuned : wpe_choice(waves);
         // with the table at most 8 chains share a compute unit: the register
         // budget of 2 waves per SIMD (1 for a launch of at most one per SIMD)

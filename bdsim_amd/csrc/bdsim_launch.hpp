@@ -15,8 +15,9 @@ namespace bdgpu {
 // float for the mixed form's FP32-factor variant (f32_factor), else double.
 // lws_bytes > 0: k_integrate_lws<MIXED> with that much dynamic LDS for the
 // hot part of the workspace (few chains; nb = 0). ptab_bytes > 0 (mixed form,
-// nb = 0, no lws): k_integrate_ptab<wpe> with that much dynamic LDS for the
-// step's pair-weight table.
+// nb = 0): that much more dynamic LDS for the step's pair-weight table, after
+// the workspace part (k_integrate_lws<true, true>) or alone
+// (k_integrate_ptab<wpe>).
 template <bool MIXED>
 void launch_integrate_small(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
                             const KArgs& a, size_t lws_bytes = 0, size_t ptab_bytes = 0);

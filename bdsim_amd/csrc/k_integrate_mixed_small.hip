@@ -9,7 +9,12 @@ template <>
 void launch_integrate_small<true>(int wpe, int nb, bool f32_factor, dim3 grid, hipStream_t stream,
                                    const KArgs& a, size_t lws_bytes, size_t ptab_bytes) {
     if (lws_bytes > 0 && nb == 0) {
-        hipLaunchKernelGGL((k_integrate_lws<true>), grid, dim3(64), lws_bytes, stream, a);
+        // ptab_bytes > 0: the table follows the workspace part in the same
+        // dynamic LDS
+        if (ptab_bytes > 0)
+            hipLaunchKernelGGL((k_integrate_lws<true, true>), grid, dim3(64), lws_bytes + ptab_bytes, stream, a);
+        else
+            hipLaunchKernelGGL((k_integrate_lws<true>), grid, dim3(64), lws_bytes, stream, a);
         return;
     }
     if (ptab_bytes > 0 && nb == 0) {
