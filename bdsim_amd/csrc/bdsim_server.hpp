@@ -416,11 +416,21 @@ int srv_service(bdsim_server* s, int gi) {
                            ? launch_split<true>((int)n, gr.split, gr.stream, a, gr.d_bar, gr.d_xbuf)
                            : launch_split<false>((int)n, gr.split, gr.stream, a, gr.d_bar, gr.d_xbuf);
         if (rc) return rc;
-    } else if (is_mixed(s->precision))
-        launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves,
-                               s->precision == BDSIM_PRECISION_MIXED_F32_FACTOR);
-    else
-        launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves);
+    } else {
+        // the launch's longest chain sizes the LDS of the 64-thread kernels
+        // (workspace part, pair-weight table)
+        int nmax = 0;
+        if (gr.block == 64)
+            for (const int32_t sl : gr.launched) {
+                auto it = s->jobs.find(s->slot_job[sl]);
+                if (it != s->jobs.end()) nmax = std::max(nmax, it->second.N);
+            }
+        if (is_mixed(s->precision))
+            launch_integrate<true>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves,
+                                   s->precision == BDSIM_PRECISION_MIXED_F32_FACTOR, nmax);
+        else
+            launch_integrate<false>(gr.block, dim3((unsigned)n), gr.stream, a, gr.nb, waves, false, nmax);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(gr.ev, gr.stream));
     gr.in_flight = true;
