@@ -656,7 +656,7 @@ struct RegBuild {
         const int mhi = min(N - 1, (16 * K + 15) / 3);
         const int W = N - mlo;
         const int np = (mhi - mlo + 1) * W;
-        __syncthreads();            // the previous block row's entries are built
+        wave_sync();                // the previous block row's entries are built
 #pragma unroll 1
         for (int p = threadIdx.x; p < np; p += 64) {
             const int a = p / W, bb = p - a * W;
@@ -668,7 +668,7 @@ struct RegBuild {
             pw[2 * (32 * a + bb)] = om1;
             pw[2 * (32 * a + bb) + 1] = om2;
         }
-        __syncthreads();
+        wave_sync();
     }
 
     // block (K, I) of block row K (after table(K)) in the accumulator layout:
@@ -762,7 +762,7 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<dou
 #pragma unroll
                     for (int I = K; I < NB; ++I) rowb[16 * I + c] = M[K][I][q];
                 }
-                __syncthreads();
+                wave_sync();
                 double piv = rowb[16 * K + jr];
                 piv = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(piv)),
                                        __builtin_amdgcn_readfirstlane(__double2loint(piv)));
@@ -771,7 +771,7 @@ __device__ __forceinline__ bool regf_block_row(mfma_d4 (&M)[NB][NB], lds_ptr<dou
                         cnt.fail_index = 16 * K + jr;
                         cnt.fail_value = piv;
                     }
-                    __syncthreads();
+                    wave_sync();
                     return false;
                 }
                 // one reciprocal square root for the pivot and the row
@@ -840,18 +840,18 @@ __device__ __forceinline__ bool dense_bdw_regs(Smem<BLOCK, MIXED>& sm, RegLds<NB
     static_assert(BLOCK == 64, "sm.R rows of 64 beads");
     const RegBuild<MIXED> build{as_lds(sm.R[0]), as_lds_mut(rlp->pw), n, N, c, g, hstar, 1.0 / hstar};
 
-    __syncthreads();
+    wave_sync();
     if (tid < N) {
         xs[3 * tid] = x[0];
         xs[3 * tid + 1] = x[1];
         xs[3 * tid + 2] = x[2];
     }
     for (int e = n + tid; e < 16 * NB; e += 64) xs[e] = 0.0;
-    __syncthreads();
+    wave_sync();
 
     mfma_d4 M[NB][NB];
     if (!regf_block_row<0, NB, MIXED>(M, yp, build, rowb, xs, sm.cnt)) return false;
-    __syncthreads();
+    wave_sync();
     y[0] = y[1] = y[2] = 0.0;
     if (tid < N)
         for (int d = 0; d < 3; ++d) {
@@ -949,6 +949,20 @@ struct Chain {
     __device__ __forceinline__ int pt_stride() const { return (N >> 1) | 1; }
 
     __device__ __forceinline__ double bsum(double v) { return block_sum<NW>(v, sm.Red, par); }
+    // the workgroup's barrier; a single-wave chain needs only the ordering of
+    // its own accesses (wave_sync), not a wait for its LDS writes
+    __device__ __forceinline__ void sync() {
+        if constexpr (NW == 1) wave_sync();
+        else __syncthreads();
+    }
+    __device__ __forceinline__ bool sync_or(bool p) {
+        if constexpr (NW == 1) {
+            wave_sync();
+            return __builtin_amdgcn_ballot_w64(p) != 0;
+        } else {
+            return __syncthreads_or(p);
+        }
+    }
     __device__ __forceinline__ void bump(int64_t& c, int64_t v = 1) { if (tid == 0) c += v; }
     __device__ __forceinline__ void bump32(int32_t& c) { if (tid == 0) c += 1; }
     template <typename T, typename U>
@@ -973,7 +987,7 @@ struct Chain {
                     const int64_t o = wl.S + (int64_t)(SLOT_SEED + d) * N + tid;
                     ws[o] = wsg[o];
                 }
-            __syncthreads();
+            sync();
         }
     }
     __device__ __forceinline__ void ws_to_hbm() {
@@ -989,9 +1003,9 @@ struct Chain {
     // Stage the mobility configuration: sm.R from the caller's R, and for the
     // mixed form its FP32 copy relative to bead 0.
     __device__ __forceinline__ void stage_R(const double R[3]) {
-        __syncthreads();
+        sync();
         for (int d = 0; d < 3; ++d) sm.R[d][tid] = R[d];
-        __syncthreads();
+        sync();
         if constexpr (S::TILES) {
             const int e = 128 * (tid >> 6) + (tid & 63);
             f32x4 r;
@@ -1005,7 +1019,7 @@ struct Chain {
             }
             sm.Pa[e].x = r.x; sm.Pa[e].y = r.y; sm.Pa[e].z = r.z;
             sm.Pa[e + 64].x = r.x; sm.Pa[e + 64].y = r.y; sm.Pa[e + 64].z = r.z;
-            __syncthreads();
+            sync();
         } else if constexpr (S::PAIRS) {
             if (own) {
                 const float rx = (float)(sm.R[0][tid] - sm.R[0][0]);
@@ -1014,7 +1028,7 @@ struct Chain {
                 sm.Pa[tid].x = rx; sm.Pa[tid].y = ry; sm.Pa[tid].z = rz;
                 sm.Pa[tid + N].x = rx; sm.Pa[tid + N].y = ry; sm.Pa[tid + N].z = rz;
             }
-            __syncthreads();
+            sync();
             if constexpr (PTAB) {
                 // the weights apply_pairs forms for the pair (mu, mu + k),
                 // the same operations on the same FP32 separations
@@ -1074,13 +1088,13 @@ struct Chain {
             xput(xb + (part * 3 + 2) * 1024 + tid, az);
         }
         __builtin_amdgcn_s_waitcnt(0);   // this wave's partials have completed
-        __syncthreads();
+        sync();
         if (tid == 0) {
             bool ok = !sync_fail;
             if (ok) ok = chain_barrier(A.bar + slot, (epoch + 1) * (uint32_t)nparts);
             sm.ScalI[6] = ok ? 1 : 0;
         }
-        __syncthreads();
+        sync();
         if (!sm.ScalI[6]) sync_fail = true;
         ++epoch;
         if (own) {
@@ -1102,7 +1116,7 @@ struct Chain {
     // pair loop has no lane-dependent select.
     __device__ __forceinline__ void apply(const double v[3], double y[3]) {
         if (!hi) { y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; return; }
-        __syncthreads();
+        sync();
         if constexpr (S::TILES) {
             const int e = 128 * (tid >> 6) + (tid & 63);
             const float vx = own ? (float)v[0] : 0.0f;
@@ -1130,7 +1144,7 @@ struct Chain {
         } else {
             for (int d = 0; d < 3; ++d) sm.X[d][tid] = v[d];
         }
-        __syncthreads();
+        sync();
         bump(sm.cnt.matvecs);
         if constexpr (S::PAIRS) {
             apply_pairs(v, y);
@@ -1520,7 +1534,7 @@ struct Chain {
                 flush_own(w, ox, oy, oz);
             }
         }
-        __syncthreads();
+        sync();
         if (own) {
             const double x0 = SPLIT ? 0.0 : v[0], x1 = SPLIT ? 0.0 : v[1], x2 = SPLIT ? 0.0 : v[2];
             y[0] = x0 + (double)sm.Acc[0][tid] * inv;
@@ -1536,9 +1550,9 @@ struct Chain {
     __device__ __forceinline__ void spring_bead_force(const double bond[3], double Fc[3], double F[3]) {
         Fc[0] = Fc[1] = Fc[2] = 0.0;
         if (tid < N - 1) connector_force(P, bond, Fc);
-        __syncthreads();
+        sync();
         for (int d = 0; d < 3; ++d) sm.V[d][tid] = Fc[d];
-        __syncthreads();
+        sync();
         for (int d = 0; d < 3; ++d) {
             double f = 0.0;
             if (tid == 0) f = sm.V[d][0];
@@ -1553,7 +1567,7 @@ struct Chain {
         F[0] = F[1] = F[2] = 0.0;
         if (P.bend_type == 0 || N == 2) return;
         const double Cst = P.bend_C;
-        __syncthreads();
+        sync();
         if (tid < N - 1) {
             double b[3];
             for (int d = 0; d < 3; ++d) b[d] = C[d][tid + 1] - C[d][tid];
@@ -1563,7 +1577,7 @@ struct Chain {
             for (int d = 0; d < 3; ++d) sm.B[d][tid] = (1.0 / len) * b[d];
             sm.B[3][tid] = len;
         }
-        __syncthreads();
+        sync();
         if (!own) return;
         const int m = tid;
         auto u = [&](int k, int d) { return sm.B[d][k]; };
@@ -1625,10 +1639,10 @@ struct Chain {
         const int et = P.ev_type;
         if (et != 1 && et != 2 && et != 3) return;
         if constexpr (MIXED) {
-            __syncthreads();
+            sync();
             if (own)
                 for (int d = 0; d < 3; ++d) sm.W[d][tid] = (float)(C[d][tid] - C[d][0]);
-            __syncthreads();
+            sync();
         }
         if (own) {
             // The self pair takes the factor's value at Rmin, finite in FP32
@@ -1823,7 +1837,7 @@ struct Chain {
             for (int d = 0; d < 3; ++d) { vprev[d] = v[d]; v[d] = w[d] / b; }
             if (tid == 0) sm.Beta[j] = b;
         }
-        __syncthreads();
+        sync();
         if (k == 1) {
             rmin = rmax = sm.Alpha[0];
             for (int d = 0; d < 3; ++d) {
@@ -1839,7 +1853,7 @@ struct Chain {
             if constexpr (RITZ32) ritz_multisect<float>(k, as_lds(sm.Alpha), as_lds(sm.Beta), sm.Scal);
             else ritz_multisect<double>(k, as_lds(sm.Alpha), as_lds(sm.Beta), sm.Scal);
         }
-        __syncthreads();
+        sync();
         rmin = sm.Scal[0];
         rmax = sm.Scal[1];
         // Ritz vectors of the two extremes by inverse iteration (lanes 0, 1)
@@ -1854,7 +1868,7 @@ struct Chain {
                                             rows + LANCZOS_COLD, rows + 2 * LANCZOS_COLD);
             }
         }
-        __syncthreads();
+        sync();
         // lift the tridiagonal eigenvectors: v = V^T y (hydrodynamics.cpp:264-273)
         double vmin[3] = {0.0, 0.0, 0.0}, vmax[3] = {0.0, 0.0, 0.0};
         if (own) {
@@ -1880,7 +1894,7 @@ struct Chain {
     __device__ __forceinline__ void cheb_coefficients(int L, double da, double db) {
         const ws_ptr coef = ws + wl.coef;
         const ws_ptr aux = ws + wl.aux;
-        __syncthreads();
+        sync();
         if constexpr (MIXED) {
             // FP32 nodes and sums: cos(pi m / (2(L+1))) = cos(2 pi t) with
             // t = (m mod 4(L+1)) / (4(L+1)) reduced exactly in integers, on
@@ -1892,7 +1906,7 @@ struct Chain {
                 const float c = __builtin_amdgcn_cosf((float)(2 * k + 1) * inv);
                 aux[k] = (double)sqrtf(c / fda - fdb / fda);
             }
-            __syncthreads();
+            sync();
             for (int j = tid; j <= L; j += BLOCK) {
                 float s = 0.0f;
                 for (int k = 0; k <= L; ++k)
@@ -1903,7 +1917,7 @@ struct Chain {
             }
         } else {
             for (int k = tid; k <= L; k += BLOCK) aux[k] = sqrt(cos(PI * (k + 0.5) / (L + 1)) / da - db / da);
-            __syncthreads();
+            sync();
             for (int j = tid; j <= L; j += BLOCK) {
                 double s = 0.0;
                 for (int k = 0; k <= L; ++k) s += aux[k] * cos(j * (k + 0.5) * PI / (L + 1));
@@ -1912,7 +1926,7 @@ struct Chain {
                 coef[j] = s;
             }
         }
-        __syncthreads();
+        sync();
     }
 
     // B.x0 by the Chebyshev method with Lanczos bounds and FD control
@@ -2054,15 +2068,15 @@ struct Chain {
             if (tid == 0) { lg = StepLog{}; lg.t = t; }
             // recenter (integrator.cpp:88-93): sequential sums, as the reference
             if (com) {
-                __syncthreads();
+                sync();
                 for (int d = 0; d < 3; ++d) sm.P[d][tid] = R[d];
-                __syncthreads();
+                sync();
                 if (tid < 3) {
                     double s = 0.0;
                     for (int mu = 0; mu < N; ++mu) s = s + sm.P[tid][mu];
                     sm.Scal[8 + tid] = (1.0 / N) * s;
                 }
-                __syncthreads();
+                sync();
                 if (own)
                     for (int d = 0; d < 3; ++d) R[d] = R[d] - sm.Scal[8 + d];
             }
@@ -2096,7 +2110,7 @@ struct Chain {
                 int32_t nix = ix, niy = iy;
                 if (own) bead_noise(as_gbl(A.jx), as_gbl(A.jy), tid, ix, iy, x0, nix, niy);
                 if (tid == N - 1) { sm.ScalI[0] = nix; sm.ScalI[1] = niy; }
-                __syncthreads();
+                sync();
                 ix = sm.ScalI[0];
                 iy = sm.ScalI[1];
                 if (!hi) {
@@ -2162,9 +2176,9 @@ struct Chain {
             // for one mat-vec fewer per step.
             double ups[3];
             {
-                __syncthreads();
+                sync();
                 for (int d = 0; d < 3; ++d) sm.P[d][tid] = Rp[d];
-                __syncthreads();
+                sync();
                 double Fep[3], Fe[3], Fs[3], DF[3], v[3], dW[3];
                 explicit_force(sm.P, t + dt, Fep);
                 load(SLOT_FE, Fe);
@@ -2189,9 +2203,9 @@ struct Chain {
                 if (tid < N - 1)
                     for (int d = 0; d < 3; ++d) cb[d] = sm.P[d][tid + 1] - Rp[d];
                 const double anchor0 = sm.P[0][0], anchor1 = sm.P[1][0], anchor2 = sm.P[2][0];
-                __syncthreads();
+                sync();
                 for (int d = 0; d < 3; ++d) sm.V[d][tid] = ups[d];
-                __syncthreads();
+                sync();
                 if (tid < N - 1)
                     for (int d = 0; d < 3; ++d) dups[d] = sm.V[d][tid + 1] - ups[d];
                 double Rprev[3] = {Rp[0], Rp[1], Rp[2]};
@@ -2204,9 +2218,9 @@ struct Chain {
                     apply(Fb, DFc);
                     bump(sm.cnt.corrector_iters);
                     bump32(lg.corrector_iters);
-                    __syncthreads();
+                    sync();
                     for (int d = 0; d < 3; ++d) sm.V[d][tid] = DFc[d];
-                    __syncthreads();
+                    sync();
                     int sst = ST_OK;
                     if (tid < N - 1) {
                         double g[3];
@@ -2216,7 +2230,7 @@ struct Chain {
                     }
                     for (int d = 0; d < 3; ++d) sm.B[d][tid] = cb[d];
                     if (sst != ST_OK) sm.ScalI[2] = sst;
-                    if (__syncthreads_or(sst != ST_OK)) { status = sm.ScalI[2]; fail = true; break; }
+                    if (sync_or(sst != ST_OK)) { status = sm.ScalI[2]; fail = true; break; }
                     // positions from the anchor by the reference's sequential sum
                     if (tid < 3) {
                         double acc = tid == 0 ? anchor0 : (tid == 1 ? anchor1 : anchor2);
@@ -2226,7 +2240,7 @@ struct Chain {
                             sm.P[tid][mu + 1] = acc;
                         }
                     }
-                    __syncthreads();
+                    sync();
                     double dv[3] = {0.0, 0.0, 0.0};
                     if (own)
                         for (int d = 0; d < 3; ++d) { R[d] = sm.P[d][tid]; dv[d] = R[d] - Rprev[d]; }
@@ -2248,7 +2262,7 @@ struct Chain {
         if (part != 0) return;      // SPLIT: part 0 writes the results
         if (own)
             for (int d = 0; d < 3; ++d) A.R[P.R_off + 3 * tid + d] = R[d];
-        __syncthreads();
+        sync();
         if (tid == 0) {
             A.rng[ti].ix = ix;
             A.rng[ti].iy = iy;

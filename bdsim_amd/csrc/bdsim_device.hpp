@@ -124,6 +124,17 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
+// Orders LDS accesses between the lanes of one wave: the LDS instructions of a
+// wave complete in the order they are issued, so a lane reads what another
+// lane of its wave wrote before, with no s_barrier and no wait for the write;
+// the wavefront-scope fences keep the compiler from moving LDS accesses across
+// this point. For workgroups of a single wave only.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Block-wide sum over NW waves. sRed holds 2*NW doubles; `par` alternates the
 // half used so consecutive calls need only one barrier each.
 template <int NW>
