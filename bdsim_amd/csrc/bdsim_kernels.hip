@@ -317,7 +317,7 @@ int dense_nb_for(int N, double hstar, int hi_method) {
 // dynamic size a launch may ask for without a kernel attribute).
 constexpr size_t LWS_MAX_TOTAL = 64 * 1024;
 constexpr size_t LDS_PER_CU = 160 * 1024;
-size_t ptab_bytes_for(int nmax, bool with_lws = false);
+size_t ptab_bytes_for(int nmax, bool with_lws = false, unsigned n_wg = 0);
 template <bool MIXED>
 size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
     static const bool enabled = [] {
@@ -337,20 +337,24 @@ size_t lws_bytes_for(int nmax, unsigned n_wg, long long waves) {
 // 64-thread chains, largest N = nmax): N * ((N / 2) | 1) float pairs, used
 // when the table and the chain's static LDS together leave room for 8 chains
 // per compute unit (N <= 51), which the two-waves-per-SIMD register budget
-// then holds; 0: the walk forms the weights itself. BDSIM_PAIR_TABLE=0 in the
-// environment turns the table off, =2 uses it at every N <= 64.
+// then holds, or at any N <= 64 when the launch's n_wg chains are resident at
+// once even so (the table then costs no occupancy the launch would use); 0:
+// the walk forms the weights itself. BDSIM_PAIR_TABLE=0 in the environment
+// turns the table off, =2 uses it at every N <= 64.
 // with_lws: the table of a small launch that keeps the workspace in LDS too
 // (k_integrate_lws, at most 4 chains per compute unit): used at every N.
 constexpr size_t PTAB_LDS_MAX = LDS_PER_CU / 8;
-size_t ptab_bytes_for(int nmax, bool with_lws) {
+size_t ptab_bytes_for(int nmax, bool with_lws, unsigned n_wg) {
     static const int mode = [] {
         const char* e = std::getenv("BDSIM_PAIR_TABLE");
         return e ? std::atoi(e) : 1;
     }();
     if (mode == 0 || nmax < 2 || nmax > 64) return 0;
     const size_t bytes = sizeof(f32x2) * (size_t)nmax * (size_t)((nmax >> 1) | 1);
-    if (!with_lws && mode != 2 && sizeof(Smem<64, true>) + bytes > PTAB_LDS_MAX) return 0;
-    return bytes;
+    const size_t per_wg = sizeof(Smem<64, true>) + bytes;
+    if (with_lws || mode == 2 || per_wg <= PTAB_LDS_MAX) return bytes;
+    // fewer chains per compute unit than 8: only when the launch fits anyway
+    return (size_t)n_wg <= (size_t)g.n_cu * (LDS_PER_CU / per_wg) ? bytes : 0;
 }
 
 // Workgroup width per block size set by the caller (bdsim_tune_width): the
@@ -402,7 +406,7 @@ void launch_integrate(int block, dim3 grid, hipStream_t stream, const KArgs& a, 
     const int tuned = ti >= 0 ? g_tune[ti].load() : 0;
     if (block == 64) {
         const size_t lws = tuned ? 0 : lws_bytes_for<MIXED>(nmax, grid.x, waves);
-        const size_t pt = (MIXED && !tuned) ? ptab_bytes_for(nmax, lws > 0) : 0;
+        const size_t pt = (MIXED && !tuned) ? ptab_bytes_for(nmax, lws > 0, grid.x) : 0;
         const int wpe = tuned ? tuned : wpe_choice(waves);
         // with the table at most 8 chains share a compute unit: the register
         // budget of 2 waves per SIMD (1 for a launch of at most one per SIMD)

@@ -97,14 +97,16 @@ def test_tune_workgroups_per_long_chain(gpu, orc):
 
 
 def test_pair_weight_table_matches_the_walk(gpu):
-    """Mixed-form chains of up to 51 beads run on k_integrate_ptab (the step's
-    RPY weights in an LDS table, formed once per step); an explicit register
-    budget for the 64-thread bucket selects k_integrate, whose walk forms the
-    weights itself. The table holds the same FP32 operations' results, so the
-    two agree bit for bit, positions and generator states, at every N from 2
-    to 51 (odd and even: the half step k = N/2), with and without HI."""
+    """Mixed-form 64-thread chains run on k_integrate_ptab (the step's RPY
+    weights in an LDS table, formed once per step): up to 51 beads always,
+    up to 64 when the launch's chains are resident at once anyway (as these
+    60 are); an explicit register budget for the 64-thread bucket selects
+    k_integrate, whose walk forms the weights itself. The table holds the
+    same FP32 operations' results, so the two agree bit for bit, positions
+    and generator states, at every N from 2 to 64 (odd and even: the half
+    step k = N/2), with and without HI."""
     rs = np.random.default_rng(11)
-    Ns = np.array([2, 3, 4, 5, 10, 17, 30, 31, 32, 49, 50, 51] * 4, dtype=np.int32)
+    Ns = np.array([2, 3, 4, 5, 10, 17, 30, 31, 32, 49, 50, 51, 52, 57, 64] * 4, dtype=np.int32)
     R, phys = [], []
     for i, n in enumerate(Ns):
         R.append(W._walk(rs, int(n)))
